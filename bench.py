@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Batched programmable-bootstrap throughput on one MI355X.
+
+Workload (one "step"): the many-LUT programmable bootstrap of 16384 LWE
+ciphertexts -- rotation exponents, collapsed blind rotation, X^(-b) rotation
+and extraction of 4 LWEs per ciphertext -- at n=750, k=1, N=2048, l=3,
+log2 Bg=7, collapse 3, torus32, 4-bit signed messages, 4 LUTs (sign,
+ReLU-clip, two steps) packed per test polynomial.  Keys and inputs are
+synthetic (seeded) and resident in HBM before the timed region.
+
+Prints one JSON line (see the repository's bench contract).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "blind-spiking-lstm_amd")
+
+# workload parameters (batched PBS throughput configuration)
+CFG = dict(B=16384, n=750, N=2048, L=3, log_beta=7, collapse=3, n_lut=4, msg_bits=4, log_q=32,
+           sigma_lwe=2.0 ** 17, sigma_bsk=2.0 ** 5)
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def fp64_flops_per_bootstrap(n, N, L, c):
+    """Algorithmic FP64 work of one collapsed blind rotation in the folded
+    N/2-point transform convention: (n/c)*[(2l+2)*5*(N/2)*log2(N/2) +
+    16*l*N*(2^c+1)]."""
+    M = N // 2
+    return (n // c) * ((2 * L + 2) * 5 * M * math.log2(M) + 16 * L * N * (2 ** c + 1))
+
+
+MANY_LUT_FNS = (
+    lambda x: 1 if x >= 0 else 0,          # sign
+    lambda x: min(max(x, 0), 7),           # ReLU clipped to 3 bits
+    lambda x: 1 if x >= -4 else 0,         # step at -4
+    lambda x: 1 if x >= 4 else 0,          # step at +4
+)
+
+
+def setup(cfg, device):
+    import numpy as np
+    import torch
+    sys.path.insert(0, PKG)
+    from cuTFHE.multiply import prepare_bsk
+    from nn_tfhe.utils_spike import many_lut_polynomial
+    from schemas.RLWE_jax import bsk_to_fourier, get_boostrapping_key_collapse
+
+    rng = np.random.default_rng(0)
+    q = 2.0 ** cfg["log_q"]
+    s = rng.integers(0, 2, cfg["n"])
+    S = rng.choice([0.0, 1.0], size=cfg["N"])
+    params = {"q": q, "beta_bs": 2 ** cfg["log_beta"], "l_bs": cfg["L"], "sigma": cfg["sigma_bsk"],
+              "degree": cfg["N"]}
+    BSK = get_boostrapping_key_collapse(S, s, params, rng, cfg["collapse"])
+    bsk = prepare_bsk(torch.as_tensor(bsk_to_fourier(BSK)).to(device), cfg["n"], cfg["collapse"])
+    del BSK
+    B = cfg["B"]
+    half = 1 << (cfg["log_q"] - 1)
+    delta = 1 << (cfg["log_q"] - cfg["msg_bits"] - 1)
+    msg = rng.integers(-(1 << (cfg["msg_bits"] - 1)), 1 << (cfg["msg_bits"] - 1), B)
+    a = rng.integers(-half, half, size=(B, cfg["n"]), dtype=np.int64)
+    e = np.rint(rng.standard_normal(B) * cfg["sigma_lwe"]).astype(np.int64)
+    b = a @ s + msg * delta + e
+    b = ((b + half) % (2 * half)) - half
+    T = many_lut_polynomial(cfg["N"], MANY_LUT_FNS[: cfg["n_lut"]], cfg["msg_bits"], float(delta))
+    lut = np.stack([np.zeros(cfg["N"]), T])
+    inputs = {
+        "pk": torch.as_tensor(a.astype(np.float64)).to(device),
+        "b": torch.as_tensor(b.astype(np.float64)).reshape(B, 1).to(device),
+        "lut": torch.as_tensor(np.broadcast_to(lut, (B, 2, cfg["N"])).copy()).to(torch.int64).to(device),
+        "bsk": bsk,
+    }
+    host = {"S": S, "msg": msg, "delta": delta}
+    return inputs, host
+
+
+def run_gpu(args):
+    import numpy as np
+    import torch
+    sys.path.insert(0, PKG)
+    from cuTFHE.multiply import mul_bootstrap, rotation_indices
+    from schemas.RLWE_jax import rotate_extract
+
+    cfg = dict(CFG)
+    cfg["B"] = args.batch
+    device = torch.device("cuda:0")
+    inputs, host = setup(cfg, device)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+
+    def step(br_events=None):
+        idx = rotation_indices(inputs["pk"], cfg["log_q"], cfg["N"], cfg["collapse"], cfg["n_lut"])
+        if br_events is not None:
+            br_events[0].record(stream)
+        br = mul_bootstrap(inputs["lut"], inputs["bsk"], None, idx, cfg["log_beta"], cfg["log_q"],
+                           cfg["L"])
+        if br_events is not None:
+            br_events[1].record(stream)
+        return rotate_extract(br, inputs["b"], cfg["log_q"], cfg["n_lut"])
+
+    for _ in range(args.warmup):
+        out = step()
+    torch.cuda.synchronize()
+    br_ms = []
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = step(evs[k])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    br_ms = [a.elapsed_time(b) for a, b in evs]
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    B = cfg["B"]
+    value = B / (ms_per_step / 1e3)
+
+    # decrypt a sample of the outputs on the host (sanity, not the parity test)
+    masks, bodies = out
+    ns = min(B, 512)
+    mk = masks[:ns].cpu().numpy().astype(np.int64)
+    bd = bodies[:ns].cpu().numpy().astype(np.int64)
+    ph = bd - mk @ host["S"].astype(np.int64)
+    ph = ((ph + 2 ** 31) % 2 ** 32) - 2 ** 31
+    dec = np.rint(ph / host["delta"]).astype(np.int64)
+    exp = np.array([[f(int(x)) for f in MANY_LUT_FNS[: cfg["n_lut"]]] for x in host["msg"][:ns]])
+    ok_frac = float((np.mod(dec, 32) == np.mod(exp, 32)).mean())
+
+    br_avg_ms = float(np.mean(br_ms))
+    flops = fp64_flops_per_bootstrap(cfg["n"], cfg["N"], cfg["L"], cfg["collapse"]) * B
+    achieved = flops / (br_avg_ms / 1e3) / 1e12
+    res = {
+        "metric": "bootstraps/s @N=2048 (batched many-LUT PBS, plain FP64-FFT form)",
+        "value": round(value, 2),
+        "unit": "bootstraps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded keys and ciphertexts)",
+        "config": {"workload": "batched many-LUT PBS", "batch": B, "n": cfg["n"], "N": cfg["N"],
+                   "l": cfg["L"], "log2_Bg": cfg["log_beta"], "collapse": cfg["collapse"],
+                   "n_lut": cfg["n_lut"], "torus": cfg["log_q"], "form": "plain_fp64"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "br_plain_kernel<2048,3>",
+                     "kernel_avg_ms": round(br_avg_ms, 3)},
+        "decrypt_ok_sample": ok_frac,
+    }
+    return res
+
+
+def cpu_baseline_child(sample: int):
+    """Oracle (NumPy FP64 restatement of the reference algorithm) on a
+    bounded sample of the same workload, single-threaded."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import tfhe_oracle as O
+    cfg = CFG
+    rng = np.random.default_rng(1)
+    N, n, L, c = cfg["N"], cfg["n"], cfg["L"], cfg["collapse"]
+    n_iter = n // c
+    m = 2 ** c
+    # random FFT-domain key of the right shape: the timing does not depend on
+    # the key values
+    bsk_k = (rng.standard_normal((n_iter, 2, L, 2, m, N // 2)) +
+             1j * rng.standard_normal((n_iter, 2, L, 2, m, N // 2))) * 2.0 ** 30
+    a = rng.integers(-2 ** 31, 2 ** 31, size=(sample, n))
+    b = rng.integers(-2 ** 31, 2 ** 31, size=sample)
+    lut = rng.integers(-2 ** 31, 2 ** 31, size=(sample, 2, N))
+    t0 = time.perf_counter()
+    idx = O.rotation_indices(a, 32, N, c, cfg["n_lut"])
+    br = O.blind_rotate_f64(lut, bsk_k, idx, cfg["log_beta"], 32, L)
+    O.cuboot_merge_tail(br, b, 32, cfg["n_lut"])
+    dt = time.perf_counter() - t0
+    print(json.dumps({"value": sample / dt, "seconds": dt}))
+
+
+def cpu_baseline(sample: int):
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+        env[k] = "1"
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+                        str(sample)], env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-400:]}
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": round(d["value"], 4), "unit": "bootstraps/s", "cores": 1, "kind": "port",
+            "sample": f"{sample} ciphertexts of the same workload through the NumPy FP64 "
+                      f"restatement of the reference algorithm (oracle), {d['seconds']:.1f} s, "
+                      f"1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=CFG["B"])
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-child", type=int, default=0)
+    args = ap.parse_args()
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args.cpu_baseline_child)
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the engine has no "
+              f"multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    res = run_gpu(args)
+    res["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,425 @@
+// Collapsed (key-unrolled) blind rotation on gfx950, plain single-launch FP64
+// form: one workgroup per ciphertext runs every step of the loop.
+//
+// Semantics are those of the reference kernel (csrc/mul_bootstrap.cuh:20-136):
+// per step g
+//   dec[bc]  = signed gadget digits of the accumulator (csrc/dec_fft.cuh:4-31),
+//              bc = 2*level + component
+//   prod[a]  = sum_bc (sum_p BSK[g,a,bc,p] * FFT(X^idx[g,p])) * FFT(dec[bc])
+//   ct[a]    = round_half_even(centred_mod(IFFT(prod[a]), q))
+//
+// Design (MI355X-first, not a translation):
+//  * Folded transform: each real length-N polynomial is one N/2-point complex
+//    FFT (u_k = (p_k - i p_{k+N/2}) zeta^k), half the work of the N-point
+//    complex transform on real data.
+//  * T = N/8 threads; each thread owns the coefficients k = t + i*T
+//    (i < 8) of both ciphertext rows, in registers, for the whole loop.  The
+//    first forward radix pass reads them directly and the last inverse pass
+//    writes them back, so the accumulator never visits LDS.
+//  * Stockham radix-4 passes (plus one radix-2 pass when log2(N/2) is odd)
+//    exchange through a 2 x N/2 complex LDS buffer, two digit polynomials at
+//    a time; after each pair's last pass a thread pulls the four bins it
+//    multiplies (r, N/2-1-r, r+T, N/2-1-r-T) into registers.
+//  * The bins r and N/2-1-r are the reference's FFT bins 2r and 2r+1, so the
+//    BSK (reference layout) streams as 32 contiguous bytes per lane.
+//  * Roots of unity come from a quarter-wave cosine table in LDS (N/2+1
+//    doubles), filled once per workgroup with the device cospi: no 2N x N/2
+//    rotation table is read.
+#include "capi_internal.h"
+#include "tfhe_device.h"
+
+namespace bslstm {
+
+template <int N>
+struct Fold {
+    static constexpr int M = N / 2;      // complex points per polynomial
+    static constexpr int T = N / 8;      // threads per ciphertext
+    static constexpr int LOG2M = __builtin_ctz(M);
+    static constexpr bool ODD = (LOG2M & 1) != 0;   // one radix-2 pass needed
+};
+
+__device__ __forceinline__ void dft4(cplx x[4], bool inv)
+{
+    cplx t0 = cadd(x[0], x[2]);
+    cplx t1 = csub(x[0], x[2]);
+    cplx t2 = cadd(x[1], x[3]);
+    cplx t3 = csub(x[1], x[3]);
+    x[0] = cadd(t0, t2);
+    x[2] = csub(t0, t2);
+    cplx r = inv ? cmul_pi(t3) : cmul_mi(t3);
+    x[1] = cadd(t1, r);
+    x[3] = csub(t1, r);
+}
+
+// Twiddle then radix-4 butterfly for Stockham span Ns (inputs j + r*M/4).
+template <int N>
+__device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, const double *ctab)
+{
+    if (Ns > 1) {
+        const int base = (Fold<N>::M / Ns) * k;     // zeta exponent of w^k, w = exp(-2 pi i/(4 Ns))
+#pragma unroll
+        for (int r = 1; r < 4; r++) {
+            cplx w = zeta_pow<N>(ctab, inv ? -r * base : r * base);
+            x[r] = cmul(x[r], w);
+        }
+    }
+    dft4(x, inv);
+}
+
+// Forward transform of two polynomials held in registers (v[p][r] = point
+// t + r*T); result left in buf[p*M + bin] in natural bin order.
+template <int N>
+__device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const double *ctab)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T;
+    const int t = threadIdx.x;
+    int Ns;
+    if constexpr (Fold<N>::ODD) {
+        // radix-2, span 1: butterflies j = t (points t, t+2T) and j = t+T (t+T, t+3T)
+        cplx o[2][4];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            o[p][0] = cadd(v[p][0], v[p][2]);
+            o[p][1] = csub(v[p][0], v[p][2]);
+            o[p][2] = cadd(v[p][1], v[p][3]);
+            o[p][3] = csub(v[p][1], v[p][3]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            buf[p * M + 2 * t] = o[p][0];
+            buf[p * M + 2 * t + 1] = o[p][1];
+            buf[p * M + 2 * (t + T)] = o[p][2];
+            buf[p * M + 2 * (t + T) + 1] = o[p][3];
+        }
+        __syncthreads();
+        Ns = 2;
+    } else {
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+            dft4(v[p], false);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                buf[p * M + 4 * t + r] = v[p][r];
+        __syncthreads();
+        Ns = 4;
+    }
+    for (; Ns < M; Ns *= 4) {
+        cplx x[2][4];
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                x[p][r] = buf[p * M + t + r * T];
+        const int k = t & (Ns - 1);
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+            r4_twiddle<N>(x[p], k, Ns, false, ctab);
+        __syncthreads();
+        const int idxD = (t / Ns) * 4 * Ns + k;
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                buf[p * M + idxD + r * Ns] = x[p][r];
+        __syncthreads();
+    }
+}
+
+// Inverse (unnormalised) transform of buf[p*M + bin]; the result lands in
+// registers, v[p][r] = point t + r*T.
+template <int N>
+__device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const double *ctab)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T;
+    const int t = threadIdx.x;
+    constexpr int R4_END = Fold<N>::ODD ? M / 2 : M;   // radix-4 passes cover spans up to R4_END/4
+    for (int Ns = 1; Ns < R4_END; Ns *= 4) {
+        cplx x[2][4];
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                x[p][r] = buf[p * M + t + r * T];
+        const int k = t & (Ns - 1);
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+            r4_twiddle<N>(x[p], k, Ns, true, ctab);
+        if (!Fold<N>::ODD && Ns * 4 == M) {
+            // last pass: idxD == t, outputs at t + r*T
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    v[p][r] = x[p][r];
+            return;
+        }
+        __syncthreads();
+        const int idxD = (t / Ns) * 4 * Ns + k;
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                buf[p * M + idxD + r * Ns] = x[p][r];
+        __syncthreads();
+    }
+    if constexpr (Fold<N>::ODD) {
+        // radix-2, span M/2: butterflies j = t and j = t+T, inputs j and j+M/2
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int j = t + h * T;
+                cplx x0 = buf[p * M + j];
+                cplx x1 = buf[p * M + j + M / 2];
+                x1 = cmul(x1, zeta_pow<N>(ctab, -4 * j));   // conj(w^j), w = exp(-2 pi i/M)
+                v[p][h] = cadd(x0, x1);                    // point j
+                v[p][h + 2] = csub(x0, x1);                // point j + M/2
+            }
+        }
+    }
+}
+
+template <int N, int L>
+__global__ __launch_bounds__(N / 8) void br_plain_kernel(const int64_t *__restrict__ ct_init,
+                                                         int64_t ct_stride,
+                                                         const cplx *__restrict__ bsk,
+                                                         const int32_t *__restrict__ indices,
+                                                         int64_t *__restrict__ out, int n_iter,
+                                                         int m, int log2_beta, int log2_q)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T, BC = 2 * L;
+    __shared__ cplx buf[2 * M];
+    __shared__ double ctab[N / 2 + 1];
+
+    const int t = threadIdx.x;
+    fill_cos_table<N>(ctab);
+
+    // accumulator rows in registers: x[c][i] = ct[c*N + t + i*T]
+    int64_t x[2][8];
+    const int64_t *src = ct_init + (size_t)blockIdx.x * ct_stride;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            x[c][i] = src[c * N + t + i * T];
+    __syncthreads();
+
+    // twist factors zeta^k for the thread's folded points k = t + r*T
+    cplx tw[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        tw[r] = zeta_pow<N>(ctab, t + r * T);
+
+    // decomposition offset (dec_fft.cuh:7-17)
+    uint64_t off = 0;
+    for (int i = 1; i <= L; i++)
+        off += 1ULL << (log2_q - i * log2_beta + log2_beta - 1);
+    {
+        const int rs = log2_q - L * log2_beta - 1;
+        if (rs >= 0)
+            off += 1ULL << rs;
+    }
+    const uint64_t dmask = (1ULL << log2_beta) - 1;
+    const int64_t half_beta = 1LL << (log2_beta - 1);
+
+    // U-bins this thread multiplies
+    const int bins[4] = {t, M - 1 - t, t + T, M - 1 - t - T};
+
+    const double q = ldexp(1.0, log2_q);
+    const double inv_q = ldexp(1.0, -log2_q);
+    const double inv_m = 1.0 / (double)M;
+    const size_t step_stride = (size_t)2 * BC * m * M;
+
+    for (int g = 0; g < n_iter; g++) {
+        // ---- decomposition + forward transforms, one level (A,B pair) at a time
+        cplx dec[BC][4];
+#pragma unroll
+        for (int lev = 0; lev < L; lev++) {
+            const int sh = log2_q - (lev + 1) * log2_beta;
+            cplx v[2][4];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int64_t lo = (int64_t)((((uint64_t)x[c][r] + off) >> sh) & dmask) - half_beta;
+                    const int64_t hi = (int64_t)((((uint64_t)x[c][r + 4] + off) >> sh) & dmask) - half_beta;
+                    // (lo - i*hi) * zeta^k
+                    const cplx u = {(double)lo, -(double)hi};
+                    v[c][r] = cmul(u, tw[r]);
+                }
+            }
+            forward_pair<N>(v, buf, ctab);
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    dec[2 * lev + c][s] = buf[c * M + bins[s]];
+        }
+
+        // ---- external product with the collapsed key (mul_bootstrap.cuh:42-67)
+        const cplx *bsk_g = bsk + (size_t)g * step_stride;
+        const int32_t *id = indices + ((size_t)blockIdx.x * n_iter + g) * m;
+        cplx prod[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                prod[a][s] = {0.0, 0.0};
+
+        for (int p = 0; p < m; p++) {
+            const int e = id[p];
+            cplx rot[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                rot[s] = zeta_pow<N>(ctab, e * (4 * bins[s] + 1));
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+#pragma unroll
+                for (int pr = 0; pr < 2; pr++) {
+                    const cplx *bp = bsk_g + (((size_t)a * BC) * m + p) * M + 2 * (t + pr * T);
+                    cplx acc0 = {0.0, 0.0}, acc1 = {0.0, 0.0};
+#pragma unroll
+                    for (int bc = 0; bc < BC; bc++) {
+                        const cplx b0 = bp[(size_t)bc * m * M];
+                        const cplx b1 = bp[(size_t)bc * m * M + 1];
+                        acc0 = cfma(b0, dec[bc][2 * pr], acc0);
+                        acc1 = cfma_conj(b1, dec[bc][2 * pr + 1], acc1);
+                    }
+                    prod[a][2 * pr] = cfma(rot[2 * pr], acc0, prod[a][2 * pr]);
+                    prod[a][2 * pr + 1] = cfma(rot[2 * pr + 1], acc1, prod[a][2 * pr + 1]);
+                }
+            }
+        }
+
+        // ---- inverse transforms, untwist, centred mod q, round (mul_bootstrap.cuh:70-93)
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                buf[a * M + bins[s]] = prod[a][s];
+        __syncthreads();
+        cplx w[2][4];
+        inverse_pair<N>(w, buf, ctab);
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const cplx z = cmul(w[a][r], cconj(tw[r]));
+                double lo = z.re * inv_m;
+                double hi = -z.im * inv_m;
+                lo = lo - q * rint(lo * inv_q);
+                hi = hi - q * rint(hi * inv_q);
+                lo = rint(lo);
+                hi = rint(hi);
+                if (log2_q >= 64) {   // +2^63 is the class of -2^63
+                    lo = lo >= 0x1p63 ? -0x1p63 : lo;
+                    hi = hi >= 0x1p63 ? -0x1p63 : hi;
+                }
+                x[a][r] = (int64_t)lo;
+                x[a][r + 4] = (int64_t)hi;
+            }
+        }
+    }
+
+    int64_t *dst = out + (size_t)blockIdx.x * 2 * N;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            dst[c * N + t + i * T] = x[c][i];
+}
+
+template <int N, int L>
+static int launch_plain(hipStream_t stream, const int64_t *ct, int64_t ct_stride, const double *bsk,
+                        const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m,
+                        int log2_beta, int log2_q)
+{
+    hipLaunchKernelGGL((br_plain_kernel<N, L>), dim3((unsigned)batch), dim3(N / 8), 0, stream, ct,
+                       ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out, n_iter, m,
+                       log2_beta, log2_q);
+    return launch_status("blind rotation");
+}
+
+template <int N>
+static int dispatch_l(int64_t l, hipStream_t s, const int64_t *ct, int64_t st, const double *bsk,
+                      const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m, int lb,
+                      int lq)
+{
+    switch (l) {
+    case 1: return launch_plain<N, 1>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    case 2: return launch_plain<N, 2>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    case 3: return launch_plain<N, 3>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    case 4: return launch_plain<N, 4>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    default: return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported l");
+    }
+}
+
+}  // namespace bslstm
+
+using namespace bslstm;
+
+extern "C" int bslstm_mul_bootstrap(void *stream, const int64_t *ct_init, int64_t ct_elems,
+                                    const double *bsk, const int64_t bsk_dims[6],
+                                    const double *all_rot_fft, int64_t rot_rows, int64_t rot_cols,
+                                    const int32_t *indices, int64_t idx_elems, int64_t *output,
+                                    int64_t batch, int64_t log2_beta, int64_t log2_q, int64_t l,
+                                    int64_t degree)
+{
+    clear_error();
+    switch (degree) {
+    case 128: case 256: case 512: case 1024: case 2048: case 4096: break;
+    default: return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    }
+    if (l < 1 || l > 4)
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported l");
+    if (!bsk_dims)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bsk_dims is NULL");
+    const int64_t n_iter = bsk_dims[0], a_size = bsk_dims[1], b_size = bsk_dims[2],
+                  c_size = bsk_dims[3], m = bsk_dims[4], d_size = bsk_dims[5];
+    if (a_size != 2 || c_size != 2 || b_size != l || d_size != degree / 2)
+        return fail(BSLSTM_INVALID_ARGUMENT,
+                    "bsk trailing dims (%lld,%lld,%lld,%lld,%lld,%lld) do not match "
+                    "(n_iter,2,l=%lld,2,m,N/2=%lld)",
+                    (long long)n_iter, (long long)a_size, (long long)b_size, (long long)c_size,
+                    (long long)m, (long long)d_size, (long long)l, (long long)(degree / 2));
+    if (n_iter < 1 || m < 1 || m > 64)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bsk n_iter=%lld, m=%lld out of range",
+                    (long long)n_iter, (long long)m);
+    if (log2_beta < 1 || log2_q < 2 || log2_q > 64 || l * log2_beta > log2_q)
+        return fail(BSLSTM_INVALID_ARGUMENT, "need 1 <= log2_beta, l*log2_beta <= log2_q <= 64");
+    if (batch < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "negative batch");
+    int64_t stride;
+    if (ct_elems == 2 * degree)
+        stride = 0;
+    else if (ct_elems == batch * 2 * degree)
+        stride = 2 * degree;
+    else
+        return fail(BSLSTM_INVALID_ARGUMENT,
+                    "ct_init has %lld elements; expected 2N=%lld or batch*2N=%lld",
+                    (long long)ct_elems, (long long)(2 * degree), (long long)(batch * 2 * degree));
+    if (idx_elems != batch * n_iter * m)
+        return fail(BSLSTM_INVALID_ARGUMENT, "indices has %lld elements; expected %lld",
+                    (long long)idx_elems, (long long)(batch * n_iter * m));
+    if (all_rot_fft && (rot_rows != 2 * degree || rot_cols != degree / 2))
+        return fail(BSLSTM_INVALID_ARGUMENT, "all_rot_fft must be (2N, N/2)");
+    if (batch == 0)
+        return BSLSTM_OK;
+    if (!ct_init || !bsk || !indices || !output)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int ni = (int)n_iter, mm = (int)m, lb = (int)log2_beta, lq = (int)log2_q;
+    switch (degree) {
+    case 128: return dispatch_l<128>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 256: return dispatch_l<256>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 512: return dispatch_l<512>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 1024: return dispatch_l<1024>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 2048: return dispatch_l<2048>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
+    default: return dispatch_l<4096>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
+    }
+}

@@ -1,0 +1,240 @@
+// gfx950 kernels around the blind rotation: rotation exponents, the many-LUT
+// rotate + sample-extract tail, and the LWE key switch.  In the reference
+// these are XLA fusions generated from JAX code; here each is one
+// hand-written, coalesced launch.
+#include <algorithm>
+
+#include "capi_internal.h"
+
+namespace bslstm {
+
+// ---------------------------------------------------------------------------
+// rotation exponents (cuTFHE/multiply.py:67-79)
+// ---------------------------------------------------------------------------
+__global__ void rotation_indices_kernel(const double *__restrict__ pk, int64_t total, int n,
+                                        int c, int m, int n_iter, double two_n, double q_nlut,
+                                        double n_lut, int two_N, int32_t *__restrict__ out)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total)
+        return;
+    const int p = (int)(gid % m);
+    const int64_t bg = gid / m;
+    const int g = (int)(bg % n_iter);
+    const int64_t b = bg / n_iter;
+    const double *row = pk + b * n + (int64_t)g * c;
+    // jnp.dot(pk (..., n_iter, c), kron.T): bit j of pattern p is MSB-first
+    double val = 0.0;
+    for (int j = 0; j < c; j++)
+        val += row[j] * (double)((p >> (c - 1 - j)) & 1);
+    const double tq = (two_n * val) / q_nlut;
+    const int32_t v = (int32_t)(rint(tq) * n_lut);
+    int32_t r = v % two_N;
+    if (r < 0)
+        r += two_N;
+    out[gid] = r;
+}
+
+// ---------------------------------------------------------------------------
+// many-LUT tail: X^(-b_ms) rotation fused with sample extraction
+// (schemas/RLWE_jax.py:295-303, text_jax.py:191-208, polynomial_jax.py:150-169)
+// ---------------------------------------------------------------------------
+__global__ void rotate_extract_kernel(const int64_t *__restrict__ br, const double *__restrict__ body,
+                                      int N, int n_lut, double two_n, double q_nlut, double nl,
+                                      double *__restrict__ masks, double *__restrict__ bodies)
+{
+    const int i = blockIdx.y;      // ciphertext
+    const int k = blockIdx.z;      // extracted coefficient / sub-LUT
+    const int64_t *A = br + (size_t)i * 2 * N;
+    const int64_t *Bp = A + N;
+    // alpha = -b_ms; the rotated polynomial R[s] = P[w] if w < N else -P[w-N],
+    // w = (s - alpha) mod 2N
+    const double bms = rint((two_n * body[i]) / q_nlut) * nl;
+    const int alpha = -(int)bms;
+    const int twoN = 2 * N;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+        // sample extract: a'_j = R_A[k-j] (j <= k) else -R_A[k-j+N]
+        int s = k - j;
+        int sign = 1;
+        if (s < 0) {
+            s += N;
+            sign = -1;
+        }
+        int w = (s - alpha) % twoN;
+        if (w < 0)
+            w += twoN;
+        int64_t v;
+        if (w < N) {
+            v = A[w];
+        } else {
+            v = -A[w - N];
+        }
+        masks[((size_t)i * n_lut + k) * N + j] = (double)(sign * v);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int w = (k - alpha) % twoN;
+        if (w < 0)
+            w += twoN;
+        const int64_t v = w < N ? Bp[w] : -Bp[w - N];
+        bodies[(size_t)i * n_lut + k] = (double)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LWE key switch (schemas/LWE_jax.py:97-117, gadget_product_LWE :65-82)
+//
+// out[r, j] = sum_{lev,i} d[r,lev,i] * KSK[lev,i,j], j in [0, n_out]; column
+// n_out is the body key.  FP64 products and sums are exact for torus32
+// (|sum| < 2^53), as in the reference's float64 tensordot.  Each workgroup
+// owns RB ciphertexts x 256 output columns; digits are staged in LDS and the
+// key streams coalesced along j.
+// ---------------------------------------------------------------------------
+constexpr int KS_RB = 32;          // ciphertexts per workgroup
+constexpr int KS_LDS_DOUBLES = 8192; // digit tile: l_ks * KS_RB * ic <= 64 KiB
+constexpr int KS_LMAX = 64;        // max decomposition levels
+constexpr int KS_THREADS = 256;
+
+__global__ __launch_bounds__(KS_THREADS) void key_switch_kernel(
+    const double *__restrict__ a_in, const double *__restrict__ b_in, const double *__restrict__ ksk_a,
+    const double *__restrict__ ksk_b, int batch, int n_in, int n_out, double beta, int l_ks, int ic_max,
+    double q, double inv_q, double *__restrict__ a_out, double *__restrict__ b_out)
+{
+    extern __shared__ double dig[];   // [l_ks][KS_RB][ic_max]
+    const int j = blockIdx.x * KS_THREADS + threadIdx.x;
+    const int r0 = blockIdx.y * KS_RB;
+    const bool active = j <= n_out;
+    double acc[KS_RB];
+#pragma unroll
+    for (int r = 0; r < KS_RB; r++)
+        acc[r] = 0.0;
+
+    for (int i0 = 0; i0 < n_in; i0 += ic_max) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < KS_RB * ic_max; e += KS_THREADS) {
+            const int r = e / ic_max, ii = e % ic_max;
+            const int row = r0 + r, i = i0 + ii;
+            double x = (row < batch && i < n_in) ? a_in[(size_t)row * n_in + i] / q : 0.0;
+            for (int lev = 0; lev < l_ks; lev++) {
+                const double xb = x * beta;
+                const double d = rint(xb);
+                x = xb - d;
+                dig[(lev * KS_RB + r) * ic_max + ii] = d;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const int ic = min(ic_max, n_in - i0);
+            for (int lev = 0; lev < l_ks; lev++) {
+                for (int ii = 0; ii < ic; ii++) {
+                    const size_t kidx = (size_t)lev * n_in + i0 + ii;
+                    const double kv = j < n_out ? ksk_a[kidx * n_out + j] : ksk_b[kidx];
+                    const double *dr = dig + (lev * KS_RB) * ic_max + ii;
+#pragma unroll
+                    for (int r = 0; r < KS_RB; r++)
+                        acc[r] = fma(dr[r * ic_max], kv, acc[r]);
+                }
+            }
+        }
+    }
+    if (!active)
+        return;
+#pragma unroll
+    for (int r = 0; r < KS_RB; r++) {
+        const int row = r0 + r;
+        if (row >= batch)
+            break;
+        double v = acc[r] - q * rint(acc[r] * inv_q);
+        if (j < n_out) {
+            a_out[(size_t)row * n_out + j] = v;
+        } else {
+            double bb = b_in[row] + v;
+            b_out[row] = bb - q * rint(bb * inv_q);
+        }
+    }
+}
+
+}  // namespace bslstm
+
+using namespace bslstm;
+
+extern "C" int bslstm_rotation_indices(void *stream, const double *public_key, int64_t batch,
+                                       int64_t n, int64_t collapse, int64_t log2_q, int64_t degree,
+                                       int64_t n_lut, int32_t *indices)
+{
+    clear_error();
+    if (collapse < 1 || collapse > 6 || n % collapse != 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "n=%lld must be a multiple of collapse=%lld in [1,6]",
+                    (long long)n, (long long)collapse);
+    if (!is_pow2(degree) || degree > (1 << 20))
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (n_lut < 1 || log2_q < 1 || log2_q > 64 || batch < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad n_lut/log2_q/batch");
+    const int n_iter = (int)(n / collapse), m = 1 << collapse;
+    const int64_t total = batch * n_iter * m;
+    if (total == 0)
+        return BSLSTM_OK;
+    if (!public_key || !indices)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const double q = ldexp(1.0, (int)log2_q);
+    const int threads = 256;
+    const int64_t blocks = (total + threads - 1) / threads;
+    hipLaunchKernelGGL(rotation_indices_kernel, dim3((unsigned)blocks), dim3(threads), 0,
+                       (hipStream_t)stream, public_key, total, (int)n, (int)collapse, m, n_iter,
+                       (double)(2 * degree), q * (double)n_lut, (double)n_lut, (int)(2 * degree),
+                       indices);
+    return launch_status("rotation indices");
+}
+
+extern "C" int bslstm_rotate_extract(void *stream, const int64_t *br_out, const double *body,
+                                     int64_t batch, int64_t degree, int64_t log2_q, int64_t n_lut,
+                                     double *masks, double *bodies)
+{
+    clear_error();
+    if (!is_pow2(degree) || degree > 65536)
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (n_lut < 1 || n_lut > degree || log2_q < 1 || log2_q > 64 || batch < 0 || batch > 65535)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad n_lut/log2_q/batch (batch <= 65535 per call)");
+    if (batch == 0)
+        return BSLSTM_OK;
+    if (!br_out || !body || !masks || !bodies)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const double q = ldexp(1.0, (int)log2_q);
+    const int threads = 256;
+    const int bx = (int)((degree + threads - 1) / threads);
+    hipLaunchKernelGGL(rotate_extract_kernel, dim3(bx, (unsigned)batch, (unsigned)n_lut),
+                       dim3(threads), 0, (hipStream_t)stream, br_out, body, (int)degree, (int)n_lut,
+                       (double)(2 * degree), q * (double)n_lut, (double)n_lut, masks, bodies);
+    return launch_status("rotate/extract");
+}
+
+extern "C" int bslstm_key_switch_lwe(void *stream, const double *a_in, const double *b_in,
+                                     const double *ksk_a, const double *ksk_b, int64_t batch,
+                                     int64_t n_in, int64_t n_out, int64_t log2_beta_ks, int64_t l_ks,
+                                     int64_t log2_q, double *a_out, double *b_out)
+{
+    clear_error();
+    if (log2_q < 1 || log2_q > 32)
+        return fail(BSLSTM_INVALID_ARGUMENT, "key switch needs log2_q <= 32 (exact FP64 sums)");
+    if (l_ks < 1 || l_ks > KS_LMAX || log2_beta_ks < 1 || l_ks * log2_beta_ks > 64)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad l_ks/log2_beta_ks");
+    if (n_in < 1 || n_out < 1 || batch < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad shapes");
+    // exactness bound: l*n_in*(beta/2)*(q/2) < 2^53
+    const double bound = (double)l_ks * (double)n_in * ldexp(1.0, (int)log2_beta_ks - 1) *
+                         ldexp(1.0, (int)log2_q - 1);
+    if (bound >= 0x1p53)
+        return fail(BSLSTM_INVALID_ARGUMENT, "key-switch sums could exceed 2^53");
+    if (batch == 0)
+        return BSLSTM_OK;
+    if (!a_in || !b_in || !ksk_a || !ksk_b || !a_out || !b_out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const double q = ldexp(1.0, (int)log2_q);
+    dim3 grid((unsigned)((n_out + 1 + KS_THREADS - 1) / KS_THREADS),
+              (unsigned)((batch + KS_RB - 1) / KS_RB));
+    const int ic_max = (int)std::min<int64_t>(32, KS_LDS_DOUBLES / (l_ks * KS_RB));
+    const size_t lds = sizeof(double) * (size_t)l_ks * KS_RB * ic_max;
+    hipLaunchKernelGGL(key_switch_kernel, grid, dim3(KS_THREADS), lds, (hipStream_t)stream, a_in,
+                       b_in, ksk_a, ksk_b, (int)batch, (int)n_in, (int)n_out,
+                       ldexp(1.0, (int)log2_beta_ks), (int)l_ks, ic_max, q, 1.0 / q, a_out, b_out);
+    return launch_status("key switch");
+}

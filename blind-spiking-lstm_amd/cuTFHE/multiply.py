@@ -1,0 +1,110 @@
+"""Drop-in for the reference's cuTFHE/multiply.py on a PyTorch-ROCm host.
+
+Same function names, argument order and meaning as cuTFHE/multiply.py:17-92;
+arrays are torch tensors on the GPU instead of jax arrays.  Every call runs
+hand-written HIP kernels through the C ABI (include/bslstm_tfhe.h) on the
+tensor's current stream; nothing falls back to the CPU.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _core
+
+
+def _as_f64(x: torch.Tensor) -> torch.Tensor:
+    return x if x.dtype == torch.float64 else x.to(torch.float64)
+
+
+def rotation_indices(public_key: torch.Tensor, log_q: int, degree: int, collapse: int,
+                     n_lut: int = 1) -> torch.Tensor:
+    """Rotation exponents of cuTFHE/multiply.py:67-79 on the GPU.
+
+    public_key (..., n) -> int32 (..., n//collapse, 2**collapse):
+    idx = int32(rint(2N*val/(q*n_lut))*n_lut) mod 2N, val = pk groups . bits.
+    """
+    _core.require_device(public_key)
+    pk = _as_f64(public_key).contiguous()
+    n = pk.shape[-1]
+    if n % collapse:
+        raise _core.TFHEInvalidArgument(f"n={n} is not a multiple of collapse={collapse}")
+    batch = pk.numel() // n if n else 0
+    out = torch.empty(*pk.shape[:-1], n // collapse, 2 ** collapse, dtype=torch.int32,
+                      device=pk.device)
+    _core.check(_core.lib().bslstm_rotation_indices(_core.stream_of(pk), pk.data_ptr(), batch, n,
+                                                    collapse, log_q, degree, n_lut,
+                                                    out.data_ptr()))
+    return out
+
+
+def prepare_bsk(BSK: torch.Tensor, n: int, collapse: int) -> torch.Tensor:
+    """BSK reshape/permute of cuTFHE/multiply.py:81-86, materialised once.
+
+    (2[pk/b], 2[L/R], l, n_eq, N/2) complex128 -> contiguous
+    (n_iter, 2[a], l, 2[L/R], 2^collapse, N/2).  A 6-D tensor is taken to be
+    in kernel layout already and is returned (contiguous) as is.
+    """
+    if BSK.dim() == 6:
+        return BSK.contiguous()
+    n_iter = n // collapse
+    shape = list(BSK.shape)
+    shape[3] = n_iter
+    shape.insert(4, 2 ** collapse)
+    return BSK.reshape(shape).permute(3, 0, 2, 1, 4, 5).contiguous()
+
+
+def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft: torch.Tensor | None,
+                  indices: torch.Tensor, log_beta: int, log_q: int, l: int) -> torch.Tensor:
+    """The raw custom call of cuTFHE/multiply.py:17-51 (handler
+    csrc/mul_bootstrap.cuh:192-243): int64 ct (…,2,N) -> int64 (batch,2,N)."""
+    devs = [ct_init, bsk_reshaped, indices] + ([all_rot_fft] if all_rot_fft is not None else [])
+    _core.require_device(*devs)
+    if ct_init.dtype != torch.int64 or indices.dtype != torch.int32:
+        raise _core.TFHEInvalidArgument("ct_init must be int64 and indices int32")
+    if bsk_reshaped.dtype != torch.complex128 or bsk_reshaped.dim() < 6:
+        raise _core.TFHEInvalidArgument("bsk must be complex128 with 6 trailing dims")
+    degree = ct_init.shape[-1]
+    ct = ct_init.contiguous()
+    bsk = bsk_reshaped.contiguous()
+    idx = indices.contiguous()
+    dims = [int(d) for d in bsk.shape[-6:]]
+    n_iter, m = dims[0], dims[4]
+    batch = idx.numel() // (n_iter * m) if n_iter * m else 0
+    out = torch.empty(batch, 2, degree, dtype=torch.int64, device=ct.device)
+    arr = (_core._I64 * 6)(*dims)
+    rot = None if all_rot_fft is None else all_rot_fft.contiguous()
+    rot_rows, rot_cols = (0, 0) if rot is None else (rot.shape[-2], rot.shape[-1])
+    _core.check(_core.lib().bslstm_mul_bootstrap(
+        _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr,
+        _core.ptr(rot), rot_rows, rot_cols, idx.data_ptr(), idx.numel(), out.data_ptr(), batch,
+        log_beta, log_q, l, degree))
+    return out
+
+
+def multiply_seq_monomial(ciphertext, public_key: torch.Tensor, BSK: torch.Tensor, log_q: int,
+                          log_beta: int, l: int, degree: int, collapse: int,
+                          all_rot_fft: torch.Tensor | None, n_lut: int) -> torch.Tensor:
+    """Blind rotation of a test polynomial by X^{sum a_i s_i}
+    (cuTFHE/multiply.py:54-92).
+
+    ciphertext   (…, 2, N) (or a (A, B) pair) test polynomial(s); one per
+                 public key or a single one broadcast to all
+    public_key   (…, n) LWE masks
+    BSK          collapsed FFT-domain key, reference layout (2,2,l,n_eq,N/2)
+                 or kernel layout (n_iter,2,l,2,2^c,N/2)
+    Returns float64 (…, 2, N) like the reference.
+    """
+    ct_init = torch.stack(tuple(ciphertext)) if isinstance(ciphertext, (tuple, list)) else ciphertext
+    n = public_key.shape[-1]
+    idx = rotation_indices(public_key, log_q, degree, collapse, n_lut)
+    bsk = prepare_bsk(BSK, n, collapse)
+    out = mul_bootstrap(ct_init.to(torch.int64), bsk, all_rot_fft, idx, log_beta, log_q, l)
+    return out.reshape(*public_key.shape[:-1], 2, degree).to(torch.float64)
+
+
+def log2_int(v: float) -> int:
+    """int(math.log2(v) + 1e-6), the rounding the reference applies to q and
+    beta before the call (schemas/RLWE_jax.py:292)."""
+    return int(math.log2(v) + 1e-6)

@@ -1,0 +1,82 @@
+"""Polynomial helpers of the reference's schemas/polynomial_jax.py.
+
+Host-side (NumPy) pieces serve key generation and encryption; torch pieces
+run on the GPU.  The module name is kept so that import lines written for
+the reference resolve unchanged.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+def centered_mod(x, q):
+    """x - q*rint(x/q) (polynomial_jax.py:7-9); numpy or torch input."""
+    if isinstance(x, torch.Tensor):
+        return x - q * torch.round(x / q)
+    return x - q * np.rint(x / q)
+
+
+def _twist(N: int, sign: float) -> np.ndarray:
+    k = np.arange(N)
+    return np.cos(np.pi * k / N) + sign * 1j * np.sin(np.pi * k / N)
+
+
+def jax_fourier(p) -> np.ndarray:
+    """Negacyclic transform, first N/2 bins of FFT(exp(-i pi k/N) p)
+    (polynomial_jax.py:11-15)."""
+    p = np.asarray(p, dtype=np.float64)
+    N = p.shape[-1]
+    return np.fft.fft(_twist(N, -1.0) * p, axis=-1)[..., : N // 2]
+
+
+def jax_inversefourier(F) -> np.ndarray:
+    """Inverse of jax_fourier with the Hermitian extension, rounded
+    (polynomial_jax.py:17-21)."""
+    F = np.asarray(F)
+    full = np.concatenate([F, np.flip(np.conj(F), axis=-1)], axis=-1)
+    n = full.shape[-1]
+    return np.round(np.real(_twist(n, 1.0) * np.fft.ifft(full, axis=-1)))
+
+
+def negacyclic_mul_small(a, s, log_q: int) -> np.ndarray:
+    """Exact a(X)*s(X) mod (X^N+1, 2^log_q) for a key s with |s_i| <= 1.
+
+    The role of the rounded FFT product the reference uses in encryption
+    (fft_polynomial_multiply, polynomial_jax.py:26-43), made exact: a is cut
+    into 16-bit limbs so every limb product stays below 2^(16+log2 N) and the
+    FP64 transform rounds it exactly; limbs are recombined mod 2^64.
+    """
+    a = np.asarray(a, dtype=np.int64)
+    s = np.asarray(s, dtype=np.float64)
+    fs = jax_fourier(s)
+    au = a.astype(np.uint64)
+    acc = np.zeros(a.shape, dtype=np.uint64)
+    for limb in range(4):
+        part = ((au >> np.uint64(16 * limb)) & np.uint64(0xFFFF)).astype(np.float64)
+        prod = jax_inversefourier(jax_fourier(part) * fs).astype(np.int64)
+        acc += prod.astype(np.uint64) << np.uint64(16 * limb)
+    q_mask = np.uint64((1 << log_q) - 1) if log_q < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    r = (acc & q_mask).astype(np.int64) if log_q < 64 else acc.astype(np.int64)
+    if log_q < 64:
+        half = 1 << (log_q - 1)
+        r = np.where(r >= half, r - (1 << log_q), r)
+    return r
+
+
+def GLEV_polynomial(x, beta: float, l: int, q: float) -> np.ndarray:
+    """x * q/beta^i for i = 1..l (polynomial_jax.py:115-134)."""
+    basis = np.array([q / (beta ** i) for i in range(1, l + 1)])[:, None]
+    return np.asarray(x, np.float64) * basis
+
+
+def coef_rotation(polynomial: torch.Tensor, alpha: torch.Tensor) -> torch.Tensor:
+    """polynomial * X^alpha mod X^N+1 (polynomial_jax.py:150-169), batched
+    over the leading axis with one integer exponent per row."""
+    N = polynomial.shape[-1]
+    alpha = alpha.to(torch.int64).reshape(-1, 1)
+    j = torch.arange(N, device=polynomial.device).reshape(1, -1)
+    w = torch.remainder(j - alpha, 2 * N)
+    src = torch.where(w < N, w, w - N)
+    sign = torch.where(w < N, 1, -1).to(polynomial.dtype)
+    return torch.gather(polynomial.reshape(-1, N), 1, src.expand(polynomial.reshape(-1, N).shape[0], N)) * sign

@@ -1,0 +1,252 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: bit-exact for integer work (rotation indices, rotate/extract, key
+switch); for the FP64 blind rotation, every output coefficient within a
+stated distance of the exact int128 oracle and decrypted messages identical
+to the exact oracle's.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import exact as X
+from oracle import tfhe_oracle as O
+from tests.tfhe_cases import decrypt_outputs, expected_mod, make_case
+
+pytestmark = pytest.mark.gpu
+
+# FP64 blind-rotation tolerance: max |GPU - exact| per coefficient after the
+# whole loop, in integer units mod q.  The FP64 transform's own rounding gives
+# O(1) differences; the phase noise is ~2^20 and more.
+BR_COEF_TOL = 64
+
+
+def dev(x, dtype=None):
+    t = torch.as_tensor(np.ascontiguousarray(x))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda:0")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def mult():
+    from cuTFHE import multiply
+    return multiply
+
+
+def test_library_is_native(cuda_device):
+    from cuTFHE import _core
+    assert _core.lib().bslstm_abi_version() == 1
+
+
+def test_rotation_indices_bit_exact(cuda_device, mult):
+    rng = np.random.default_rng(11)
+    for (N, n, c, n_lut) in [(2048, 750, 3, 4), (1024, 630, 3, 1), (512, 24, 2, 2), (256, 12, 4, 3)]:
+        q = 1 << 32
+        pk = rng.integers(-q // 2, q // 2, size=(64, n)).astype(np.float64)
+        # built ties: odd multiples of q*n_lut/(4N) hit x.5 exactly
+        unit = q * n_lut / (4 * N)
+        if unit == int(unit):
+            pk[:8] = (2 * rng.integers(-2000, 2000, size=(8, n)) + 1) * unit
+        ref = O.rotation_indices(pk, 32, N, c, n_lut)
+        got = host(mult.rotation_indices(dev(pk), 32, N, c, n_lut))
+        assert got.dtype == np.int32
+        np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("N,L,log_beta", [(128, 1, 10), (128, 4, 7), (256, 2, 8), (512, 3, 7),
+                                          (1024, 3, 7), (2048, 3, 7), (2048, 4, 6), (4096, 2, 9)])
+def test_mul_bootstrap_vs_exact_int128(cuda_device, mult, N, L, log_beta):
+    case = make_case(N=N, n=12, L=L, log_beta=log_beta, collapse=3, B=6, seed=N + L)
+    idx = O.rotation_indices(case.a, 32, N, 3, 1)
+    ct = np.broadcast_to(case.lut, (6, 2, N))
+    got = host(mult.mul_bootstrap(dev(ct), dev(case.bsk_fft_k), None, dev(idx), log_beta, 32, L))
+    exact = X.blind_rotate(case.lut, case.bsk_coef_k, idx, log_beta, 32, L)
+    d = O.centered_mod(got - exact, 32)
+    print(f"N={N} L={L}: max|gpu-exact|={np.abs(d).max()}")
+    assert np.abs(d).max() <= BR_COEF_TOL
+
+
+def test_mul_bootstrap_broadcast_equals_batched(cuda_device, mult):
+    case = make_case(N=1024, n=24, L=3, log_beta=7, B=8, seed=5)
+    idx = dev(O.rotation_indices(case.a, 32, 1024, 3, 1))
+    bsk = dev(case.bsk_fft_k)
+    one = host(mult.mul_bootstrap(dev(case.lut), bsk, None, idx, 7, 32, 3))
+    many = host(mult.mul_bootstrap(dev(np.broadcast_to(case.lut, (8, 2, 1024))), bsk, None, idx, 7, 32, 3))
+    np.testing.assert_array_equal(one, many)
+
+
+def test_mul_bootstrap_deterministic(cuda_device, mult):
+    case = make_case(N=2048, n=24, L=3, log_beta=7, B=4, seed=9)
+    idx = dev(O.rotation_indices(case.a, 32, 2048, 3, 1))
+    bsk = dev(case.bsk_fft_k)
+    r1 = host(mult.mul_bootstrap(dev(case.lut), bsk, None, idx, 7, 32, 3))
+    r2 = host(mult.mul_bootstrap(dev(case.lut), bsk, None, idx, 7, 32, 3))
+    np.testing.assert_array_equal(r1, r2)
+
+
+def test_mul_bootstrap_vs_fp64_restatement(cuda_device, mult):
+    """Same algorithm as the reference kernel (FP64 FFT domain), batch of 32."""
+    case = make_case(N=1024, n=30, L=3, log_beta=7, B=32, seed=21)
+    idx = O.rotation_indices(case.a, 32, 1024, 3, 1)
+    ct = np.broadcast_to(case.lut, (32, 2, 1024))
+    got = host(mult.mul_bootstrap(dev(ct), dev(case.bsk_fft_k), None, dev(idx), 7, 32, 3))
+    ref = O.blind_rotate_f64(ct, case.bsk_fft_k, idx, 7, 32, 3)
+    assert np.abs(O.centered_mod(got - ref, 32)).max() <= BR_COEF_TOL
+
+
+def test_rotate_extract_bit_exact(cuda_device):
+    from schemas.RLWE_jax import rotate_extract
+    rng = np.random.default_rng(3)
+    for N, n_lut in [(1024, 1), (2048, 4), (256, 3)]:
+        br = rng.integers(-2 ** 31, 2 ** 31, size=(33, 2, N))
+        b = rng.integers(-2 ** 31, 2 ** 31, size=33).astype(np.float64)
+        b[:4] = [0, -2 ** 31, 2 ** 31 - 1, 2 ** 20 * n_lut + 2 ** 19 * n_lut]   # incl. a tie
+        masks, bodies = rotate_extract(dev(br), dev(b), 32, n_lut)
+        rm, rb = O.cuboot_merge_tail(br, b, 32, n_lut)
+        np.testing.assert_array_equal(host(masks).astype(np.int64), rm)
+        np.testing.assert_array_equal(host(bodies).astype(np.int64), rb)
+
+
+def test_key_switch_bit_exact(cuda_device):
+    from schemas.LWE_jax import key_switch_LWE
+    rng = np.random.default_rng(4)
+    for (n_in, n_out, lb, l) in [(1024, 630, 4, 5), (2048, 750, 4, 5), (300, 37, 2, 12)]:
+        s_old = rng.integers(0, 2, n_in)
+        s_new = rng.integers(0, 2, n_out)
+        ka, kb = O.keyswitch_key(rng, s_new, s_old, 32, lb, l, 2.0 ** 12)
+        a = rng.integers(-2 ** 31, 2 ** 31, size=(70, n_in))
+        a[0, :3] = [2 ** 31 - 1, -2 ** 31, 2 ** (32 - lb - 1)]     # extremes and a tie
+        b = rng.integers(-2 ** 31, 2 ** 31, size=70)
+        ra, rb = O.key_switch_lwe(ka, kb, a, b, 32, lb, l)
+        params = {"beta_ks": 2 ** lb, "l_ks": l, "q": 2.0 ** 32}
+        ga, gb = key_switch_LWE((dev(ka, torch.float64), dev(kb[..., None], torch.float64)),
+                                (dev(a, torch.float64), dev(b[:, None], torch.float64)), params)
+        assert O.eq_mod(host(ga).astype(np.int64), ra, 32)
+        assert O.eq_mod(host(gb).astype(np.int64).reshape(-1), rb, 32)
+
+
+def test_errors_are_invalid_argument(cuda_device, mult):
+    from cuTFHE._core import TFHEInvalidArgument
+    case = make_case(N=128, n=12, L=2, log_beta=7, B=2, seed=1)
+    idx = dev(O.rotation_indices(case.a, 32, 128, 3, 1))
+    with pytest.raises(TFHEInvalidArgument, match="Unsupported l"):
+        mult.mul_bootstrap(dev(case.lut), dev(case.bsk_fft_k), None, idx, 7, 32, 5)
+    with pytest.raises(TFHEInvalidArgument):
+        mult.mul_bootstrap(dev(np.zeros((2, 100), np.int64)), dev(case.bsk_fft_k), None, idx, 7, 32, 2)
+    with pytest.raises(TFHEInvalidArgument):   # wrong index count
+        mult.mul_bootstrap(dev(case.lut), dev(case.bsk_fft_k), None, idx[:1, :2], 7, 32, 2)
+
+
+def _pbs_gpu(case, lut_batched=True, B=None):
+    from cuTFHE.multiply import log2_int  # noqa: F401
+    from schemas.RLWE_jax import cuboot_merge
+    Bn = len(case.b) if B is None else B
+    lut = np.broadcast_to(case.lut, (Bn, 2, case.N))
+    masks, bodies = cuboot_merge((dev(case.a[:Bn], torch.float64), dev(case.b[:Bn, None], torch.float64)),
+                                 [dev(lut[:, 0], torch.float64), dev(lut[:, 1], torch.float64)],
+                                 dev(case.bsk_ordered), 2.0 ** case.log_q, 2.0 ** case.log_beta, case.L,
+                                 case.N, case.collapse, None, case.n_lut)
+    return host(masks), host(bodies)
+
+
+def _pbs_exact(case, B):
+    idx = O.rotation_indices(case.a[:B], case.log_q, case.N, case.collapse, case.n_lut)
+    br = X.blind_rotate(case.lut, case.bsk_coef_k, idx, case.log_beta, case.log_q, case.L)
+    return O.cuboot_merge_tail(br, case.b[:B], case.log_q, case.n_lut)
+
+
+def test_pbs_identity_lut_exhaustive_small(cuda_device):
+    """Every 3-bit message through the identity LUT (noise-free LWE)."""
+    case = make_case(N=512, n=36, L=3, log_beta=7, B=64, msg_bits=3, seed=2, sigma_lwe=0.0,
+                     sigma_bsk=2.0 ** 6)
+    masks, bodies = _pbs_gpu(case)
+    dec, err = decrypt_outputs(case, masks, bodies[..., 0])
+    np.testing.assert_array_equal(dec, expected_mod(case))
+    assert set(case.msg.tolist()) == set(range(8))
+
+
+def test_pbs_config1_shape_vs_exact(cuda_device):
+    """N=1024, n=630, l=3, log2 Bg=7, identity LUT, 3-bit messages (the CPU
+    baseline configuration), then key switch back to n with base 2^4, l=5."""
+    from schemas.LWE_jax import key_switch_LWE
+    case = make_case(N=1024, n=630, L=3, log_beta=7, B=8, msg_bits=3, seed=7, sigma_bsk=2.0 ** 7)
+    masks, bodies = _pbs_gpu(case)
+    em, eb = _pbs_exact(case, 8)
+    dec_g, err_g = decrypt_outputs(case, masks, bodies[..., 0])
+    dec_x, err_x = decrypt_outputs(case, em, eb)
+    np.testing.assert_array_equal(dec_g, dec_x)
+    np.testing.assert_array_equal(dec_g, expected_mod(case))
+    dev_max = np.abs(O.centered_mod(masks.astype(np.int64) - em, 32)).max()
+    print(f"config-1 shape: max|gpu-exact| mask coef {dev_max}; err std gpu {err_g.std():.4g} "
+          f"exact {err_x.std():.4g}")
+    # key switch N=1024 -> n=630 and decrypt under s
+    rng = np.random.default_rng(70)
+    ka, kb = O.keyswitch_key(rng, case.s, case.S, 32, 4, 5, 2.0 ** 10)
+    params = {"beta_ks": 16, "l_ks": 5, "q": 2.0 ** 32}
+    ga, gb = key_switch_LWE((dev(ka, torch.float64), dev(kb[..., None], torch.float64)),
+                            (dev(masks[:, 0]), dev(bodies[:, 0])), params)
+    ga, gb = host(ga).astype(np.int64), host(gb).astype(np.int64).reshape(-1)
+    ra, rb = O.key_switch_lwe(ka, kb, masks[:, 0].astype(np.int64), bodies[:, 0, 0].astype(np.int64), 32, 4, 5)
+    assert O.eq_mod(ga, ra, 32) and O.eq_mod(gb, rb, 32)
+    ph = O.centered_mod(gb - ga @ case.s, 32)
+    np.testing.assert_array_equal(np.mod(np.rint(ph / case.delta_out).astype(np.int64), 16),
+                                  case.msg)
+
+
+def test_pbs_config2_shape_many_lut_vs_exact(cuda_device):
+    """N=2048, n=750, l=3, log2 Bg=7, 4-bit signed messages, 4 LUTs per test
+    polynomial: decryptions identical to the exact int128 oracle's."""
+    case = make_case(N=2048, n=750, L=3, log_beta=7, B=6, n_lut=4, msg_bits=4, seed=8,
+                     sigma_bsk=2.0 ** 5, kind="many")
+    masks, bodies = _pbs_gpu(case)
+    em, eb = _pbs_exact(case, 6)
+    dec_g, err_g = decrypt_outputs(case, masks, bodies[..., 0])
+    dec_x, err_x = decrypt_outputs(case, em, eb)
+    np.testing.assert_array_equal(dec_g, dec_x)
+    d = np.abs(O.centered_mod(masks.astype(np.int64) - em, 32)).max()
+    print(f"config-2 shape: max|gpu-exact| {d}; err std gpu {err_g.std():.4g} exact {err_x.std():.4g}")
+    assert d <= BR_COEF_TOL
+    # the exact oracle itself meets LUT(m) wherever its modulus-switch error allows
+    ok = (dec_x == expected_mod(case))
+    assert ok.mean() >= 0.9
+
+
+def test_multiply_seq_monomial_dropin(cuda_device, mult):
+    """Float64 surface with the reference-layout BSK (2,2,l,n_eq,N/2)."""
+    case = make_case(N=256, n=24, L=2, log_beta=9, B=5, seed=12)
+    pk = dev(case.a, torch.float64)
+    ct = dev(np.broadcast_to(case.lut, (5, 2, 256)), torch.float64)
+    rot = dev(np.stack([O.negacyclic_fft(np.eye(256)[r % 256] * (-1) ** (r // 256)) for r in range(512)]))
+    out = host(mult.multiply_seq_monomial(ct, pk, dev(case.bsk_ordered), 32, 9, 2, 256, 3, rot, 1))
+    assert out.dtype == np.float64 and out.shape == (5, 2, 256)
+    idx = O.rotation_indices(case.a, 32, 256, 3, 1)
+    exact = X.blind_rotate(case.lut, case.bsk_coef_k, idx, 9, 32, 2)
+    assert np.abs(O.centered_mod(out.astype(np.int64) - exact, 32)).max() <= BR_COEF_TOL
+
+
+def test_heaviside_bootstrapper_cuboot(cuda_device):
+    """KS (N -> n) then many-LUT PBS, as HeavysideBoostrapper.cuboot does
+    (nn_tfhe/activation.py:148-162)."""
+    from nn_tfhe.activation import HeavysideBoostrapper
+    case = make_case(N=512, n=48, L=3, log_beta=7, B=16, n_lut=2, msg_bits=3, seed=31,
+                     sigma_bsk=2.0 ** 5, kind="many")
+    rng = np.random.default_rng(31)
+    # inputs encrypted under the GLWE key S (dimension N), as in the LSTM
+    a_big, b_big = O.lwe_encrypt(rng, case.msg * case.delta_in, case.S, 32, 2.0 ** 12)
+    ka, kb = O.keyswitch_key(rng, case.s, case.S, 32, 4, 6, 2.0 ** 8)
+    p_lut = {"q": 2.0 ** 32, "beta_bs": 2 ** 7, "l_bs": 3, "degree": 512}
+    p_ks = {"q": 2.0 ** 32, "beta_ks": 16, "l_ks": 6, "degree": 48}
+    bs = HeavysideBoostrapper(torch.as_tensor(case.bsk_ordered), (ka.astype(np.float64),
+                              kb[..., None].astype(np.float64)), None, None, {}, p_lut, p_ks, {},
+                              3, None, 2)
+    lut = np.broadcast_to(case.lut, (16, 2, 512))
+    masks, bodies = bs.cuboot((dev(a_big, torch.float64), dev(b_big[:, None], torch.float64)),
+                              [dev(lut[:, 0], torch.float64), dev(lut[:, 1], torch.float64)])
+    dec, _ = decrypt_outputs(case, host(masks), host(bodies)[..., 0])
+    np.testing.assert_array_equal(dec, expected_mod(case))
