@@ -1,0 +1,80 @@
+"""The C-ABI library loads, exports every symbol include/bslstm_tfhe.h
+declares, and rejects bad arguments on the host before any GPU work (these
+error paths run without a GPU: validation precedes every HIP call)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from cuTFHE import _core
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "bslstm_tfhe.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(bslstm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert {"bslstm_mul_bootstrap", "bslstm_rotation_indices", "bslstm_rotate_extract",
+            "bslstm_key_switch_lwe", "bslstm_last_error", "bslstm_abi_version"} <= set(syms)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _core.lib()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+        assert name in _core.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_abi_version():
+    assert _core.lib().bslstm_abi_version() == 1
+
+
+def _mul(degree=2048, l=3, dims=(250, 2, 3, 2, 8, 1024), ct_elems=None, idx_elems=None, batch=4,
+         log2_beta=7, log2_q=32):
+    arr = (ctypes.c_int64 * 6)(*dims)
+    ct_elems = 2 * degree if ct_elems is None else ct_elems
+    idx_elems = batch * dims[0] * dims[4] if idx_elems is None else idx_elems
+    return _core.lib().bslstm_mul_bootstrap(None, 8, ct_elems, 8, arr, None, 0, 0, 8, idx_elems, 8,
+                                            batch, log2_beta, log2_q, l, degree)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(degree=100), "Unsupported degree"),
+    (dict(degree=8192), "Unsupported degree"),
+    (dict(l=5, dims=(250, 2, 5, 2, 8, 1024)), "Unsupported l"),
+    (dict(l=0), "Unsupported l"),
+    (dict(dims=(250, 2, 2, 2, 8, 1024)), "bsk trailing dims"),
+    (dict(dims=(250, 2, 3, 2, 8, 512)), "bsk trailing dims"),
+    (dict(ct_elems=3 * 2048), "ct_init has"),
+    (dict(idx_elems=7), "indices has"),
+    (dict(log2_beta=11), "log2_beta"),
+])
+def test_mul_bootstrap_rejects(kw, msg):
+    assert _mul(**kw) == _core.BSLSTM_INVALID_ARGUMENT
+    assert msg in _core.lib().bslstm_last_error().decode()
+
+
+def test_other_entry_points_reject():
+    lib = _core.lib()
+    assert lib.bslstm_rotation_indices(None, 8, 4, 10, 3, 32, 2048, 4, 8) == _core.BSLSTM_INVALID_ARGUMENT
+    assert lib.bslstm_rotate_extract(None, 8, 8, 4, 1000, 32, 4, 8, 8) == _core.BSLSTM_INVALID_ARGUMENT
+    assert lib.bslstm_key_switch_lwe(None, 8, 8, 8, 8, 4, 2048, 750, 4, 5, 64, 8, 8) == \
+        _core.BSLSTM_INVALID_ARGUMENT
+    assert "log2_q <= 32" in lib.bslstm_last_error().decode()
+
+
+def test_empty_batch_is_noop():
+    assert _mul(batch=0, idx_elems=0) == _core.BSLSTM_OK
+
+
+def test_python_surface_raises_invalid_argument():
+    import torch
+    from cuTFHE import multiply
+    with pytest.raises(_core.TFHEInvalidArgument):
+        multiply.rotation_indices(torch.zeros(3, 10, dtype=torch.float64), 32, 2048, 3, 1)
