@@ -210,7 +210,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=CFG["B"])
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one elementwise negation over this many MiB (reads and "
+                         "writes that many bytes): the calibration kernel for HBM counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
     args = ap.parse_args()
@@ -222,6 +225,13 @@ def main():
               f"multi-GPU path", file=sys.stderr)
         sys.exit(2)
     res = run_gpu(args)
+    if args.copy_mib:
+        import torch
+        src = torch.ones(args.copy_mib * 2 ** 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        res["copy_calibration_bytes"] = args.copy_mib * 2 ** 20
     res["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
     print(json.dumps(res), flush=True)
 
