@@ -97,31 +97,45 @@ def run_gpu(args):
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
 
-    def step(br_events=None):
+    def step(form, br_events=None):
         idx = rotation_indices(inputs["pk"], cfg["log_q"], cfg["N"], cfg["collapse"], cfg["n_lut"])
         if br_events is not None:
             br_events[0].record(stream)
         br = mul_bootstrap(inputs["lut"], inputs["bsk"], None, idx, cfg["log_beta"], cfg["log_q"],
-                           cfg["L"])
+                           cfg["L"], form=form)
         if br_events is not None:
             br_events[1].record(stream)
         return rotate_extract(br, inputs["b"], cfg["log_q"], cfg["n_lut"])
 
-    for _ in range(args.warmup):
-        out = step()
-    torch.cuda.synchronize()
-    br_ms = []
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        out = step(evs[k])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    br_ms = [a.elapsed_time(b) for a, b in evs]
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    def timed(form, steps, warmup):
+        for _ in range(warmup):
+            step(form)
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            out = step(form, evs[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        br_ms = [a.elapsed_time(b) for a, b in evs]
+        return out, (t1 - t0) * 1e3 / steps, float(np.mean(br_ms))
+
     B = cfg["B"]
+    forms = {}
+    if args.form in ("plain", "both"):
+        _, ms_plain, br_plain = timed("plain", max(1, args.steps // 2) if args.form == "both" else
+                                      args.steps, 1 if args.form == "both" else args.warmup)
+        forms["plain_fp64"] = {"bootstraps_per_s": round(B / (ms_plain / 1e3), 2),
+                               "ms_per_step": round(ms_plain, 3), "kernel_avg_ms": round(br_plain, 3)}
+    main_form = "plain" if args.form == "plain" else "batched"
+    if main_form == "batched":
+        out, ms_per_step, br_avg_ms = timed("batched", args.steps, args.warmup)
+        forms["batched_fp64"] = {"bootstraps_per_s": round(B / (ms_per_step / 1e3), 2),
+                                 "ms_per_step": round(ms_per_step, 3),
+                                 "kernel_avg_ms": round(br_avg_ms, 3)}
+    else:
+        out, ms_per_step, br_avg_ms = step("plain"), ms_plain, br_plain
     value = B / (ms_per_step / 1e3)
 
     # decrypt a sample of the outputs on the host (sanity, not the parity test)
@@ -135,11 +149,10 @@ def run_gpu(args):
     exp = np.array([[f(int(x)) for f in MANY_LUT_FNS[: cfg["n_lut"]]] for x in host["msg"][:ns]])
     ok_frac = float((np.mod(dec, 32) == np.mod(exp, 32)).mean())
 
-    br_avg_ms = float(np.mean(br_ms))
     flops = fp64_flops_per_bootstrap(cfg["n"], cfg["N"], cfg["L"], cfg["collapse"]) * B
     achieved = flops / (br_avg_ms / 1e3) / 1e12
     res = {
-        "metric": "bootstraps/s @N=2048 (batched many-LUT PBS, plain FP64-FFT form)",
+        "metric": "bootstraps/s @N=2048 (batched many-LUT PBS)",
         "value": round(value, 2),
         "unit": "bootstraps/s",
         "n_gpus": 1,
@@ -153,12 +166,14 @@ def run_gpu(args):
         "data": "synthetic (seeded keys and ciphertexts)",
         "config": {"workload": "batched many-LUT PBS", "batch": B, "n": cfg["n"], "N": cfg["N"],
                    "l": cfg["L"], "log2_Bg": cfg["log_beta"], "collapse": cfg["collapse"],
-                   "n_lut": cfg["n_lut"], "torus": cfg["log_q"], "form": "plain_fp64"},
+                   "n_lut": cfg["n_lut"], "torus": cfg["log_q"], "form": main_form + "_fp64"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel": "br_plain_kernel<2048,3>",
+                     "traffic": None,
+                     "kernel": f"br_kernel<2048,3,{1 if main_form == 'plain' else 2}>",
                      "kernel_avg_ms": round(br_avg_ms, 3)},
         "decrypt_ok_sample": ok_frac,
+        "forms": forms,
     }
     return res
 
@@ -210,6 +225,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=CFG["B"])
+    ap.add_argument("--form", choices=("plain", "batched", "both"), default="both",
+                    help="blind-rotation form(s) to time; the JSON value is the batched form "
+                         "unless only plain is asked for")
     ap.add_argument("--cpu-sample", type=int, default=96)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one elementwise negation over this many MiB (reads and "

@@ -1,5 +1,4 @@
-// Collapsed (key-unrolled) blind rotation on gfx950, plain single-launch FP64
-// form: one workgroup per ciphertext runs every step of the loop.
+// Collapsed (key-unrolled) blind rotation on gfx950, FP64 transform forms.
 //
 // Semantics are those of the reference kernel (csrc/mul_bootstrap.cuh:20-136):
 // per step g
@@ -12,18 +11,25 @@
 //  * Folded transform: each real length-N polynomial is one N/2-point complex
 //    FFT (u_k = (p_k - i p_{k+N/2}) zeta^k), half the work of the N-point
 //    complex transform on real data.
-//  * T = N/8 threads; each thread owns the coefficients k = t + i*T
-//    (i < 8) of both ciphertext rows, in registers, for the whole loop.  The
+//  * T = N/8 threads per ciphertext; each owns the coefficients k = t + i*T
+//    (i < 8) of both accumulator rows, in registers, for the whole loop.  The
 //    first forward radix pass reads them directly and the last inverse pass
 //    writes them back, so the accumulator never visits LDS.
 //  * Stockham radix-4 passes (plus one radix-2 pass when log2(N/2) is odd)
-//    exchange through a 2 x N/2 complex LDS buffer, two digit polynomials at
-//    a time; after each pair's last pass a thread pulls the four bins it
-//    multiplies (r, N/2-1-r, r+T, N/2-1-r-T) into registers.
-//  * The bins r and N/2-1-r are the reference's FFT bins 2r and 2r+1, so the
-//    BSK (reference layout) streams as 32 contiguous bytes per lane.
-//  * Roots of unity come from a quarter-wave cosine table in LDS (N/2+1
-//    doubles), filled once per workgroup with the device cospi: no 2N x N/2
+//    exchange through LDS, one gadget level (the A- and B-digit polynomials)
+//    at a time.
+//  * The multiply-accumulate runs per level and per bin pair (r, N/2-1-r):
+//    these U-bins are the reference's FFT bins 2r and 2r+1, so every BSK read
+//    is 32 contiguous bytes per lane; the next pattern's BSK values are
+//    prefetched while the current one is consumed, keeping eight 16-byte
+//    loads in flight per lane.
+//  * CT ciphertexts per workgroup.  CT = 1 is the plain single-launch form
+//    (one workgroup per ciphertext, as the reference).  CT = 2 is the batched
+//    form: a thread multiplies each BSK value it loads into both
+//    ciphertexts, halving the key traffic from L2.  Both run identical
+//    arithmetic per ciphertext, so their outputs are bit-identical.
+//  * Roots of unity come from a quarter-wave table in LDS (N/2 complex),
+//    filled once per workgroup with the device sincospi: no 2N x N/2
 //    rotation table is read.
 #include "capi_internal.h"
 #include "tfhe_device.h"
@@ -53,13 +59,13 @@ __device__ __forceinline__ void dft4(cplx x[4], bool inv)
 
 // Twiddle then radix-4 butterfly for Stockham span Ns (inputs j + r*M/4).
 template <int N>
-__device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, const double *ctab)
+__device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, const cplx *ztab)
 {
     if (Ns > 1) {
         const int base = (Fold<N>::M / Ns) * k;     // zeta exponent of w^k, w = exp(-2 pi i/(4 Ns))
 #pragma unroll
         for (int r = 1; r < 4; r++) {
-            cplx w = zeta_pow<N>(ctab, inv ? -r * base : r * base);
+            cplx w = zeta_pow<N>(ztab, inv ? -r * base : r * base);
             x[r] = cmul(x[r], w);
         }
     }
@@ -67,12 +73,13 @@ __device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, c
 }
 
 // Forward transform of two polynomials held in registers (v[p][r] = point
-// t + r*T); result left in buf[p*M + bin] in natural bin order.
+// t + r*T, t = thread index within the ciphertext's group); result left in
+// buf[p*M + bin] in natural bin order.  All threads of the workgroup call it
+// (it contains workgroup barriers).
 template <int N>
-__device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const double *ctab)
+__device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
-    const int t = threadIdx.x;
     int Ns;
     if constexpr (Fold<N>::ODD) {
         // radix-2, span 1: butterflies j = t (points t, t+2T) and j = t+T (t+T, t+3T)
@@ -117,7 +124,7 @@ __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const d
         const int k = t & (Ns - 1);
 #pragma unroll
         for (int p = 0; p < 2; p++)
-            r4_twiddle<N>(x[p], k, Ns, false, ctab);
+            r4_twiddle<N>(x[p], k, Ns, false, ztab);
         __syncthreads();
         const int idxD = (t / Ns) * 4 * Ns + k;
 #pragma unroll
@@ -130,12 +137,11 @@ __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const d
 }
 
 // Inverse (unnormalised) transform of buf[p*M + bin]; the result lands in
-// registers, v[p][r] = point t + r*T.
+// registers, v[p][r] = point t + r*T.  Called by all threads of the workgroup.
 template <int N>
-__device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const double *ctab)
+__device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
-    const int t = threadIdx.x;
     constexpr int R4_END = Fold<N>::ODD ? M / 2 : M;   // radix-4 passes cover spans up to R4_END/4
     for (int Ns = 1; Ns < R4_END; Ns *= 4) {
         cplx x[2][4];
@@ -147,7 +153,7 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const d
         const int k = t & (Ns - 1);
 #pragma unroll
         for (int p = 0; p < 2; p++)
-            r4_twiddle<N>(x[p], k, Ns, true, ctab);
+            r4_twiddle<N>(x[p], k, Ns, true, ztab);
         if (!Fold<N>::ODD && Ns * 4 == M) {
             // last pass: idxD == t, outputs at t + r*T
 #pragma unroll
@@ -175,7 +181,7 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const d
                 const int j = t + h * T;
                 cplx x0 = buf[p * M + j];
                 cplx x1 = buf[p * M + j + M / 2];
-                x1 = cmul(x1, zeta_pow<N>(ctab, -4 * j));   // conj(w^j), w = exp(-2 pi i/M)
+                x1 = cmul(x1, zeta_pow<N>(ztab, -4 * j));   // conj(w^j), w = exp(-2 pi i/M)
                 v[p][h] = cadd(x0, x1);                    // point j
                 v[p][h + 2] = csub(x0, x1);                // point j + M/2
             }
@@ -183,24 +189,71 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const d
     }
 }
 
-template <int N, int L>
-__global__ __launch_bounds__(N / 8) void br_plain_kernel(const int64_t *__restrict__ ct_init,
-                                                         int64_t ct_stride,
-                                                         const cplx *__restrict__ bsk,
-                                                         const int32_t *__restrict__ indices,
-                                                         int64_t *__restrict__ out, int n_iter,
-                                                         int m, int log2_beta, int log2_q)
+// BSK values of one bin pair for output row a, pattern p, gadget level lev:
+// [bc2][0/1] = F-bins 2r, 2r+1 of BSK[g, a, 2*lev+bc2, p].  The block
+// pointer is uniform across the workgroup and the lane offset (2r) is 32-bit.
+template <int L>
+__device__ __forceinline__ void load_pair(cplx (&dst)[2][2], const cplx *__restrict__ bsk_g, int r2,
+                                          int a, int lev, int p, int m, int M)
+{
+#pragma unroll
+    for (int b2 = 0; b2 < 2; b2++) {
+        const cplx *blk = bsk_g + ((size_t)(a * 2 * L + 2 * lev + b2) * m + p) * M;
+        dst[b2][0] = blk[r2];
+        dst[b2][1] = blk[r2 + 1];
+    }
+}
+
+// W[bc2][c][.] += BSK(p) * rotation root of ciphertext c, for bins r and
+// M-1-r (the latter in conjugate form, see tfhe_device.h).
+template <int N, int CT>
+__device__ __forceinline__ void mac_pattern(cplx (&W)[2][CT][2], const cplx (&bk)[2][2],
+                                            const int32_t *const (&idp)[CT], int p, int r,
+                                            const cplx *ztab)
+{
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const int e = idp[c][p];
+        const cplx rot0 = zeta_pow<N>(ztab, e * (4 * r + 1));
+        // zeta^(e(4(M-1-r)+1)) = conj(zeta^(e(4r+1)) * zeta^(2e))
+        const cplx rot1 = cconj(cmul(rot0, zeta_pow<N>(ztab, 2 * e)));
+#pragma unroll
+        for (int b2 = 0; b2 < 2; b2++) {
+            W[b2][c][0] = cfma(bk[b2][0], rot0, W[b2][c][0]);
+            W[b2][c][1] = cfma_conj(bk[b2][1], rot1, W[b2][c][1]);
+        }
+    }
+}
+
+template <int N, int L, int CT>
+__global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__restrict__ ct_init,
+                                                        int64_t ct_stride,
+                                                        const cplx *__restrict__ bsk,
+                                                        const int32_t *__restrict__ indices,
+                                                        int64_t *__restrict__ out, int64_t batch,
+                                                        int n_iter, int m, int log2_beta,
+                                                        int log2_q)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T, BC = 2 * L;
-    __shared__ cplx buf[2 * M];
-    __shared__ double ctab[N / 2 + 1];
+    constexpr int PPT = 2 / CT;                  // bin pairs per thread in the MAC
+    static_assert(CT == 1 || CT == 2, "one or two ciphertexts per workgroup");
+    __shared__ cplx buf[CT * 2 * M];             // per ciphertext: two polynomials
+    __shared__ cplx ztab[M];
 
-    const int t = threadIdx.x;
-    fill_cos_table<N>(ctab);
+    const int grp = threadIdx.x / T;             // ciphertext slot of this thread (transforms)
+    const int t = threadIdx.x - grp * T;         // thread index within the slot
+    const int u = threadIdx.x;                   // MAC unit
+    const int64_t first = (int64_t)blockIdx.x * CT;
+    int64_t ct_of[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++)
+        ct_of[c] = first + c < batch ? first + c : batch - 1;   // tail: recompute the last, store nothing
+
+    fill_root_table<N>(ztab);
 
     // accumulator rows in registers: x[c][i] = ct[c*N + t + i*T]
     int64_t x[2][8];
-    const int64_t *src = ct_init + (size_t)blockIdx.x * ct_stride;
+    const int64_t *src = ct_init + ct_of[grp] * ct_stride;
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -212,7 +265,7 @@ __global__ __launch_bounds__(N / 8) void br_plain_kernel(const int64_t *__restri
     cplx tw[4];
 #pragma unroll
     for (int r = 0; r < 4; r++)
-        tw[r] = zeta_pow<N>(ctab, t + r * T);
+        tw[r] = zeta_pow<N>(ztab, t + r * T);
 
     // decomposition offset (dec_fft.cuh:7-17)
     uint64_t off = 0;
@@ -226,19 +279,32 @@ __global__ __launch_bounds__(N / 8) void br_plain_kernel(const int64_t *__restri
     const uint64_t dmask = (1ULL << log2_beta) - 1;
     const int64_t half_beta = 1LL << (log2_beta - 1);
 
-    // U-bins this thread multiplies
-    const int bins[4] = {t, M - 1 - t, t + T, M - 1 - t - T};
-
     const double q = ldexp(1.0, log2_q);
     const double inv_q = ldexp(1.0, -log2_q);
     const double inv_m = 1.0 / (double)M;
     const size_t step_stride = (size_t)2 * BC * m * M;
+    cplx *mybuf = buf + grp * 2 * M;
 
+#pragma unroll 1
     for (int g = 0; g < n_iter; g++) {
-        // ---- decomposition + forward transforms, one level (A,B pair) at a time
-        cplx dec[BC][4];
+        const cplx *bsk_g = bsk + (size_t)g * step_stride;
+        const int32_t *idp[CT];
 #pragma unroll
+        for (int c = 0; c < CT; c++)
+            idp[c] = indices + (ct_of[c] * n_iter + g) * m;
+
+        cplx prod[2][PPT][CT][2];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int pp = 0; pp < PPT; pp++)
+#pragma unroll
+                for (int c = 0; c < CT; c++)
+                    prod[a][pp][c][0] = prod[a][pp][c][1] = {0.0, 0.0};
+
+#pragma unroll 1
         for (int lev = 0; lev < L; lev++) {
+            // ---- digits of level lev (dec_fft.cuh:19-29), fold, forward transform
             const int sh = log2_q - (lev + 1) * log2_beta;
             cplx v[2][4];
 #pragma unroll
@@ -247,50 +313,59 @@ __global__ __launch_bounds__(N / 8) void br_plain_kernel(const int64_t *__restri
                 for (int r = 0; r < 4; r++) {
                     const int64_t lo = (int64_t)((((uint64_t)x[c][r] + off) >> sh) & dmask) - half_beta;
                     const int64_t hi = (int64_t)((((uint64_t)x[c][r + 4] + off) >> sh) & dmask) - half_beta;
-                    // (lo - i*hi) * zeta^k
-                    const cplx u = {(double)lo, -(double)hi};
-                    v[c][r] = cmul(u, tw[r]);
+                    v[c][r] = cmul({(double)lo, -(double)hi}, tw[r]);   // (lo - i*hi) * zeta^k
                 }
             }
-            forward_pair<N>(v, buf, ctab);
-#pragma unroll
-            for (int c = 0; c < 2; c++)
-#pragma unroll
-                for (int s = 0; s < 4; s++)
-                    dec[2 * lev + c][s] = buf[c * M + bins[s]];
-        }
+            forward_pair<N>(v, mybuf, ztab, t);
 
-        // ---- external product with the collapsed key (mul_bootstrap.cuh:42-67)
-        const cplx *bsk_g = bsk + (size_t)g * step_stride;
-        const int32_t *id = indices + ((size_t)blockIdx.x * n_iter + g) * m;
-        cplx prod[2][4];
+            // ---- external product, level lev (mul_bootstrap.cuh:42-67)
+#pragma unroll 1
+            for (int pp = 0; pp < PPT; pp++) {
+                const int r = CT == 1 ? u + pp * T : u;      // r < M/2; partner M-1-r
+#pragma unroll 1
+                for (int a = 0; a < 2; a++) {
+                    cplx W[2][CT][2];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+                    for (int b2 = 0; b2 < 2; b2++)
 #pragma unroll
-            for (int s = 0; s < 4; s++)
-                prod[a][s] = {0.0, 0.0};
-
-        for (int p = 0; p < m; p++) {
-            const int e = id[p];
-            cplx rot[4];
-#pragma unroll
-            for (int s = 0; s < 4; s++)
-                rot[s] = zeta_pow<N>(ctab, e * (4 * bins[s] + 1));
-#pragma unroll
-            for (int a = 0; a < 2; a++) {
-#pragma unroll
-                for (int pr = 0; pr < 2; pr++) {
-                    const cplx *bp = bsk_g + (((size_t)a * BC) * m + p) * M + 2 * (t + pr * T);
-                    cplx acc0 = {0.0, 0.0}, acc1 = {0.0, 0.0};
-#pragma unroll
-                    for (int bc = 0; bc < BC; bc++) {
-                        const cplx b0 = bp[(size_t)bc * m * M];
-                        const cplx b1 = bp[(size_t)bc * m * M + 1];
-                        acc0 = cfma(b0, dec[bc][2 * pr], acc0);
-                        acc1 = cfma_conj(b1, dec[bc][2 * pr + 1], acc1);
+                        for (int c = 0; c < CT; c++)
+                            W[b2][c][0] = W[b2][c][1] = {0.0, 0.0};
+                    // ping-pong prefetch: the loads of pattern p+1 fly while p is consumed
+                    cplx bkA[2][2], bkB[2][2];
+                    load_pair<L>(bkA, bsk_g, 2 * r, a, lev, 0, m, M);
+#pragma unroll 1
+                    for (int p = 0; p < m; p += 2) {
+                        load_pair<L>(bkB, bsk_g, 2 * r, a, lev, p + 1 < m ? p + 1 : p, m, M);
+                        mac_pattern<N, CT>(W, bkA, idp, p, r, ztab);
+                        if (p + 1 < m) {
+                            load_pair<L>(bkA, bsk_g, 2 * r, a, lev, p + 2 < m ? p + 2 : p + 1, m, M);
+                            mac_pattern<N, CT>(W, bkB, idp, p + 1, r, ztab);
+                        }
                     }
-                    prod[a][2 * pr] = cfma(rot[2 * pr], acc0, prod[a][2 * pr]);
-                    prod[a][2 * pr + 1] = cfma(rot[2 * pr + 1], acc1, prod[a][2 * pr + 1]);
+                    cplx acc[CT][2];
+#pragma unroll
+                    for (int c = 0; c < CT; c++) {
+                        acc[c][0] = acc[c][1] = {0.0, 0.0};
+#pragma unroll
+                        for (int b2 = 0; b2 < 2; b2++) {
+                            const cplx d0 = buf[c * 2 * M + b2 * M + r];
+                            const cplx d1 = buf[c * 2 * M + b2 * M + M - 1 - r];
+                            acc[c][0] = cfma(W[b2][c][0], d0, acc[c][0]);
+                            acc[c][1] = cfma(W[b2][c][1], d1, acc[c][1]);
+                        }
+                    }
+                    // (a, pp) are uniform loop counters: select the accumulator
+                    // with compile-time indices so it stays in registers
+#pragma unroll
+                    for (int aa = 0; aa < 2; aa++)
+#pragma unroll
+                        for (int qq = 0; qq < PPT; qq++)
+                            if (aa == a && qq == pp)
+#pragma unroll
+                                for (int c = 0; c < CT; c++) {
+                                    prod[aa][qq][c][0] = cadd(prod[aa][qq][c][0], acc[c][0]);
+                                    prod[aa][qq][c][1] = cadd(prod[aa][qq][c][1], acc[c][1]);
+                                }
                 }
             }
         }
@@ -298,13 +373,19 @@ __global__ __launch_bounds__(N / 8) void br_plain_kernel(const int64_t *__restri
         // ---- inverse transforms, untwist, centred mod q, round (mul_bootstrap.cuh:70-93)
         __syncthreads();
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int pp = 0; pp < PPT; pp++) {
+            const int r = CT == 1 ? u + pp * T : u;
 #pragma unroll
-            for (int s = 0; s < 4; s++)
-                buf[a * M + bins[s]] = prod[a][s];
+            for (int c = 0; c < CT; c++)
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    buf[c * 2 * M + a * M + r] = prod[a][pp][c][0];
+                    buf[c * 2 * M + a * M + M - 1 - r] = prod[a][pp][c][1];
+                }
+        }
         __syncthreads();
         cplx w[2][4];
-        inverse_pair<N>(w, buf, ctab);
+        inverse_pair<N>(w, mybuf, ztab, t);
 #pragma unroll
         for (int a = 0; a < 2; a++) {
 #pragma unroll
@@ -326,51 +407,50 @@ __global__ __launch_bounds__(N / 8) void br_plain_kernel(const int64_t *__restri
         }
     }
 
-    int64_t *dst = out + (size_t)blockIdx.x * 2 * N;
+    if (first + grp < batch) {
+        int64_t *dst = out + (first + grp) * 2 * N;
 #pragma unroll
-    for (int c = 0; c < 2; c++)
+        for (int c = 0; c < 2; c++)
 #pragma unroll
-        for (int i = 0; i < 8; i++)
-            dst[c * N + t + i * T] = x[c][i];
+            for (int i = 0; i < 8; i++)
+                dst[c * N + t + i * T] = x[c][i];
+    }
 }
 
-template <int N, int L>
-static int launch_plain(hipStream_t stream, const int64_t *ct, int64_t ct_stride, const double *bsk,
-                        const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m,
-                        int log2_beta, int log2_q)
+template <int N, int L, int CT>
+static int launch_br(hipStream_t stream, const int64_t *ct, int64_t ct_stride, const double *bsk,
+                     const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m,
+                     int log2_beta, int log2_q)
 {
-    hipLaunchKernelGGL((br_plain_kernel<N, L>), dim3((unsigned)batch), dim3(N / 8), 0, stream, ct,
-                       ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out, n_iter, m,
-                       log2_beta, log2_q);
+    const int64_t blocks = (batch + CT - 1) / CT;
+    hipLaunchKernelGGL((br_kernel<N, L, CT>), dim3((unsigned)blocks), dim3(N / 8 * CT), 0, stream,
+                       ct, ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out, batch, n_iter,
+                       m, log2_beta, log2_q);
     return launch_status("blind rotation");
 }
 
-template <int N>
+template <int N, int CT>
 static int dispatch_l(int64_t l, hipStream_t s, const int64_t *ct, int64_t st, const double *bsk,
                       const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m, int lb,
                       int lq)
 {
     switch (l) {
-    case 1: return launch_plain<N, 1>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
-    case 2: return launch_plain<N, 2>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
-    case 3: return launch_plain<N, 3>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
-    case 4: return launch_plain<N, 4>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    case 1: return launch_br<N, 1, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    case 2: return launch_br<N, 2, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    case 3: return launch_br<N, 3, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    case 4: return launch_br<N, 4, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
     default: return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported l");
     }
 }
 
-}  // namespace bslstm
-
-using namespace bslstm;
-
-extern "C" int bslstm_mul_bootstrap(void *stream, const int64_t *ct_init, int64_t ct_elems,
-                                    const double *bsk, const int64_t bsk_dims[6],
-                                    const double *all_rot_fft, int64_t rot_rows, int64_t rot_cols,
-                                    const int32_t *indices, int64_t idx_elems, int64_t *output,
-                                    int64_t batch, int64_t log2_beta, int64_t log2_q, int64_t l,
-                                    int64_t degree)
+// Host-side validation shared by both forms (mul_bootstrap.cuh:138-228 plus
+// the shape checks the reference does not make).
+static int validate(int64_t ct_elems, const double *bsk, const int64_t *bsk_dims,
+                    const double *all_rot_fft, int64_t rot_rows, int64_t rot_cols,
+                    const int32_t *indices, int64_t idx_elems, int64_t *output, int64_t batch,
+                    int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree,
+                    const int64_t *ct_init, int64_t *stride)
 {
-    clear_error();
     switch (degree) {
     case 128: case 256: case 512: case 1024: case 2048: case 4096: break;
     default: return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
@@ -392,13 +472,12 @@ extern "C" int bslstm_mul_bootstrap(void *stream, const int64_t *ct_init, int64_
                     (long long)n_iter, (long long)m);
     if (log2_beta < 1 || log2_q < 2 || log2_q > 64 || l * log2_beta > log2_q)
         return fail(BSLSTM_INVALID_ARGUMENT, "need 1 <= log2_beta, l*log2_beta <= log2_q <= 64");
-    if (batch < 0)
-        return fail(BSLSTM_INVALID_ARGUMENT, "negative batch");
-    int64_t stride;
+    if (batch < 0 || batch > 0x7fffffffLL)
+        return fail(BSLSTM_INVALID_ARGUMENT, "batch out of range");
     if (ct_elems == 2 * degree)
-        stride = 0;
+        *stride = 0;
     else if (ct_elems == batch * 2 * degree)
-        stride = 2 * degree;
+        *stride = 2 * degree;
     else
         return fail(BSLSTM_INVALID_ARGUMENT,
                     "ct_init has %lld elements; expected 2N=%lld or batch*2N=%lld",
@@ -408,18 +487,58 @@ extern "C" int bslstm_mul_bootstrap(void *stream, const int64_t *ct_init, int64_
                     (long long)idx_elems, (long long)(batch * n_iter * m));
     if (all_rot_fft && (rot_rows != 2 * degree || rot_cols != degree / 2))
         return fail(BSLSTM_INVALID_ARGUMENT, "all_rot_fft must be (2N, N/2)");
-    if (batch == 0)
-        return BSLSTM_OK;
-    if (!ct_init || !bsk || !indices || !output)
+    if (batch > 0 && (!ct_init || !bsk || !indices || !output))
         return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    return BSLSTM_OK;
+}
+
+template <int CT>
+static int run(void *stream, const int64_t *ct_init, int64_t ct_elems, const double *bsk,
+               const int64_t *bsk_dims, const double *all_rot_fft, int64_t rot_rows,
+               int64_t rot_cols, const int32_t *indices, int64_t idx_elems, int64_t *output,
+               int64_t batch, int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree)
+{
+    clear_error();
+    int64_t st = 0;
+    int rc = validate(ct_elems, bsk, bsk_dims, all_rot_fft, rot_rows, rot_cols, indices, idx_elems,
+                      output, batch, log2_beta, log2_q, l, degree, ct_init, &st);
+    if (rc != BSLSTM_OK || batch == 0)
+        return rc;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const int ni = (int)n_iter, mm = (int)m, lb = (int)log2_beta, lq = (int)log2_q;
+    const int ni = (int)bsk_dims[0], mm = (int)bsk_dims[4], lb = (int)log2_beta, lq = (int)log2_q;
     switch (degree) {
-    case 128: return dispatch_l<128>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
-    case 256: return dispatch_l<256>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
-    case 512: return dispatch_l<512>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
-    case 1024: return dispatch_l<1024>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
-    case 2048: return dispatch_l<2048>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
-    default: return dispatch_l<4096>(l, s, ct_init, stride, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 128: return dispatch_l<128, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 256: return dispatch_l<256, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 512: return dispatch_l<512, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 1024: return dispatch_l<1024, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 2048: return dispatch_l<2048, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
+    default:
+        // 4096 x 2 ciphertexts would need all 160 KiB of LDS: one per workgroup
+        return dispatch_l<4096, 1>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
     }
+}
+
+}  // namespace bslstm
+
+extern "C" int bslstm_mul_bootstrap(void *stream, const int64_t *ct_init, int64_t ct_elems,
+                                    const double *bsk, const int64_t bsk_dims[6],
+                                    const double *all_rot_fft, int64_t rot_rows, int64_t rot_cols,
+                                    const int32_t *indices, int64_t idx_elems, int64_t *output,
+                                    int64_t batch, int64_t log2_beta, int64_t log2_q, int64_t l,
+                                    int64_t degree)
+{
+    return bslstm::run<1>(stream, ct_init, ct_elems, bsk, bsk_dims, all_rot_fft, rot_rows, rot_cols,
+                          indices, idx_elems, output, batch, log2_beta, log2_q, l, degree);
+}
+
+extern "C" int bslstm_mul_bootstrap_batched(void *stream, const int64_t *ct_init, int64_t ct_elems,
+                                            const double *bsk, const int64_t bsk_dims[6],
+                                            const double *all_rot_fft, int64_t rot_rows,
+                                            int64_t rot_cols, const int32_t *indices,
+                                            int64_t idx_elems, int64_t *output, int64_t batch,
+                                            int64_t log2_beta, int64_t log2_q, int64_t l,
+                                            int64_t degree)
+{
+    return bslstm::run<2>(stream, ct_init, ct_elems, bsk, bsk_dims, all_rot_fft, rot_rows, rot_cols,
+                          indices, idx_elems, output, batch, log2_beta, log2_q, l, degree);
 }

@@ -1,7 +1,6 @@
-// Device helpers shared by the gfx950 TFHE kernels: complex FP64 arithmetic,
-// the quarter-wave cosine table that serves every root of unity the
-// negacyclic transform needs, and the folded N/2-point negacyclic transform
-// conventions.
+// Device helpers shared by the gfx950 TFHE kernels: complex FP64 arithmetic
+// and the quarter-wave root table that serves every root of unity the
+// negacyclic transform needs.
 //
 // Transform convention (the reference's, csrc/dec_fft.cuh:33-57 and
 // schemas/polynomial_jax.py:11-21): for a real polynomial p of degree < N,
@@ -52,31 +51,32 @@ __device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
 __device__ __forceinline__ cplx cmul_mi(cplx a) { return {a.im, -a.re}; }
 __device__ __forceinline__ cplx cmul_pi(cplx a) { return {-a.im, a.re}; }
 
-// Quarter-wave table ctab[x] = cos(pi*x/N), x in [0, N/2], in LDS.
+// Root table in LDS: ztab[r] = zeta^r = exp(-i*pi*r/N) for r in [0, N/2).
 template <int N>
-__device__ __forceinline__ double cos_pi_over_n(const double *ctab, int e)
+__device__ __forceinline__ void fill_root_table(cplx *ztab)
 {
-    // e in [0, 2N)
-    int e1 = e <= N ? e : 2 * N - e;                 // cos is even about pi
-    double v = ctab[e1 <= N / 2 ? e1 : N - e1];
-    return e1 <= N / 2 ? v : -v;
+    for (int r = threadIdx.x; r < N / 2; r += blockDim.x) {
+        double s, c;
+        sincospi((double)r / (double)N, &s, &c);
+        ztab[r] = {c, -s};
+    }
 }
 
-// zeta^e = exp(-i*pi*e/N) for any integer e.
+// zeta^e for any integer e: zeta^(N/2) = -i, so zeta^e = (-i)^q * ztab[e mod N/2]
+// with q = (e mod 2N) div (N/2).  One 16-byte LDS read.
 template <int N>
-__device__ __forceinline__ cplx zeta_pow(const double *ctab, int e)
+__device__ __forceinline__ cplx zeta_pow(const cplx *ztab, int e)
 {
     e &= 2 * N - 1;
-    double c = cos_pi_over_n<N>(ctab, e);
-    double s = cos_pi_over_n<N>(ctab, (e - N / 2) & (2 * N - 1));   // sin(pi e/N)
-    return {c, -s};
-}
-
-template <int N>
-__device__ __forceinline__ void fill_cos_table(double *ctab)
-{
-    for (int x = threadIdx.x; x <= N / 2; x += blockDim.x)
-        ctab[x] = cospi((double)x / (double)N);
+    const int q = e / (N / 2);
+    const cplx z = ztab[e & (N / 2 - 1)];
+    double re = (q & 1) ? z.im : z.re;
+    double im = (q & 1) ? -z.re : z.im;
+    if (q & 2) {
+        re = -re;
+        im = -im;
+    }
+    return {re, im};
 }
 
 }  // namespace bslstm

@@ -55,10 +55,20 @@ def prepare_bsk(BSK: torch.Tensor, n: int, collapse: int) -> torch.Tensor:
     return BSK.reshape(shape).permute(3, 0, 2, 1, 4, 5).contiguous()
 
 
+FORMS = ("plain", "batched")
+
+
 def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft: torch.Tensor | None,
-                  indices: torch.Tensor, log_beta: int, log_q: int, l: int) -> torch.Tensor:
+                  indices: torch.Tensor, log_beta: int, log_q: int, l: int,
+                  form: str = "batched") -> torch.Tensor:
     """The raw custom call of cuTFHE/multiply.py:17-51 (handler
-    csrc/mul_bootstrap.cuh:192-243): int64 ct (…,2,N) -> int64 (batch,2,N)."""
+    csrc/mul_bootstrap.cuh:192-243): int64 ct (…,2,N) -> int64 (batch,2,N).
+
+    form "plain": one workgroup per ciphertext (the reference's launch
+    shape); "batched": two ciphertexts per workgroup sharing every key load.
+    Both give bit-identical results."""
+    if form not in FORMS:
+        raise _core.TFHEInvalidArgument(f"form must be one of {FORMS}")
     devs = [ct_init, bsk_reshaped, indices] + ([all_rot_fft] if all_rot_fft is not None else [])
     _core.require_device(*devs)
     if ct_init.dtype != torch.int64 or indices.dtype != torch.int32:
@@ -76,7 +86,9 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     arr = (_core._I64 * 6)(*dims)
     rot = None if all_rot_fft is None else all_rot_fft.contiguous()
     rot_rows, rot_cols = (0, 0) if rot is None else (rot.shape[-2], rot.shape[-1])
-    _core.check(_core.lib().bslstm_mul_bootstrap(
+    entry = _core.lib().bslstm_mul_bootstrap if form == "plain" else \
+        _core.lib().bslstm_mul_bootstrap_batched
+    _core.check(entry(
         _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr,
         _core.ptr(rot), rot_rows, rot_cols, idx.data_ptr(), idx.numel(), out.data_ptr(), batch,
         log_beta, log_q, l, degree))

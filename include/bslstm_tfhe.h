@@ -68,6 +68,22 @@ int bslstm_mul_bootstrap(void *stream,
                          int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree);
 
 /*
+ * Blind rotation, batched FP64 form: same operands, attributes, semantics and
+ * errors as bslstm_mul_bootstrap, and bit-identical outputs; each workgroup
+ * runs two ciphertexts and feeds every bootstrapping-key value it loads into
+ * both, halving the key traffic from L2.  This is the engine's form of the
+ * reference handler's work (csrc/mul_bootstrap.cuh:97-243); degree 4096 runs
+ * one ciphertext per workgroup.
+ */
+int bslstm_mul_bootstrap_batched(void *stream,
+                                 const int64_t *ct_init, int64_t ct_elems,
+                                 const double *bsk, const int64_t bsk_dims[6],
+                                 const double *all_rot_fft, int64_t rot_rows, int64_t rot_cols,
+                                 const int32_t *indices, int64_t idx_elems,
+                                 int64_t *output, int64_t batch,
+                                 int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree);
+
+/*
  * Rotation exponents of the collapsed blind rotation.
  * Replaces the index preparation of cuTFHE/multiply.py:67-79 (an XLA fusion
  * in the reference):

@@ -59,17 +59,35 @@ def test_rotation_indices_bit_exact(cuda_device, mult):
         np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("form", ["plain", "batched"])
 @pytest.mark.parametrize("N,L,log_beta", [(128, 1, 10), (128, 4, 7), (256, 2, 8), (512, 3, 7),
                                           (1024, 3, 7), (2048, 3, 7), (2048, 4, 6), (4096, 2, 9)])
-def test_mul_bootstrap_vs_exact_int128(cuda_device, mult, N, L, log_beta):
-    case = make_case(N=N, n=12, L=L, log_beta=log_beta, collapse=3, B=6, seed=N + L)
+def test_mul_bootstrap_vs_exact_int128(cuda_device, mult, N, L, log_beta, form):
+    case = make_case(N=N, n=12, L=L, log_beta=log_beta, collapse=3, B=7, seed=N + L)
     idx = O.rotation_indices(case.a, 32, N, 3, 1)
-    ct = np.broadcast_to(case.lut, (6, 2, N))
-    got = host(mult.mul_bootstrap(dev(ct), dev(case.bsk_fft_k), None, dev(idx), log_beta, 32, L))
+    ct = np.broadcast_to(case.lut, (7, 2, N))
+    got = host(mult.mul_bootstrap(dev(ct), dev(case.bsk_fft_k), None, dev(idx), log_beta, 32, L,
+                                  form=form))
     exact = X.blind_rotate(case.lut, case.bsk_coef_k, idx, log_beta, 32, L)
     d = O.centered_mod(got - exact, 32)
-    print(f"N={N} L={L}: max|gpu-exact|={np.abs(d).max()}")
+    print(f"N={N} L={L} {form}: max|gpu-exact|={np.abs(d).max()}")
     assert np.abs(d).max() <= BR_COEF_TOL
+
+
+@pytest.mark.parametrize("N,L,c", [(2048, 3, 3), (1024, 2, 2), (512, 4, 1), (256, 3, 4)])
+def test_plain_and_batched_forms_bit_identical(cuda_device, mult, N, L, c):
+    """Odd batch (a tail workgroup with one ciphertext) and per-ciphertext
+    test polynomials; collapse 1..4 (m = 2..16 patterns)."""
+    case = make_case(N=N, n=12 * c, L=L, log_beta=7, collapse=c, B=9, seed=3 * N + c)
+    rng = np.random.default_rng(c)
+    cts = rng.integers(-2 ** 31, 2 ** 31, size=(9, 2, N))
+    idx = dev(O.rotation_indices(case.a, 32, N, c, 1))
+    bsk = dev(case.bsk_fft_k)
+    r_plain = host(mult.mul_bootstrap(dev(cts), bsk, None, idx, 7, 32, L, form="plain"))
+    r_batch = host(mult.mul_bootstrap(dev(cts), bsk, None, idx, 7, 32, L, form="batched"))
+    np.testing.assert_array_equal(r_plain, r_batch)
+    exact = X.blind_rotate(cts, case.bsk_coef_k, host(idx), 7, 32, L)
+    assert np.abs(O.centered_mod(r_batch - exact, 32)).max() <= BR_COEF_TOL
 
 
 def test_mul_bootstrap_broadcast_equals_batched(cuda_device, mult):
