@@ -28,9 +28,11 @@
 //    form: a thread multiplies each BSK value it loads into both
 //    ciphertexts, halving the key traffic from L2.  Both run identical
 //    arithmetic per ciphertext, so their outputs are bit-identical.
-//  * Roots of unity come from a quarter-wave table in LDS (N/2 complex),
+//  * Roots of unity come from a half-period table in LDS (N complex),
 //    filled once per workgroup with the device sincospi: no 2N x N/2
 //    rotation table is read.
+#include <type_traits>
+
 #include "capi_internal.h"
 #include "tfhe_device.h"
 
@@ -189,43 +191,213 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const c
     }
 }
 
-// BSK values of one bin pair for output row a, pattern p, gadget level lev:
-// [bc2][0/1] = F-bins 2r, 2r+1 of BSK[g, a, 2*lev+bc2, p].  The block
-// pointer is uniform across the workgroup and the lane offset (2r) is 32-bit.
-template <int L>
-__device__ __forceinline__ void load_pair(cplx (&dst)[2][2], const cplx *__restrict__ bsk_g, int r2,
-                                          int a, int lev, int p, int m, int M)
+// ---------------------------------------------------------------------------
+// Wave-local transform for N in {512, 1024, 2048}: W = N/512 waves per
+// ciphertext, each wave owning a 256-point region of the N/2-point transform.
+// One radix-W stage crosses waves (a register butterfly plus one LDS
+// exchange); the remaining four radix-4 Stockham passes run inside a wave,
+// ordered by the wave's own in-order LDS traffic, with no workgroup barrier.
+// Bin b of the result sits at phys(b) = (b mod W)*256 + b/W.
+// ---------------------------------------------------------------------------
+template <int N>
+struct Wave {
+    static constexpr int W = N / 512;
+    static constexpr bool OK = (N >= 512 && N <= 2048);
+};
+
+template <int N>
+__device__ __forceinline__ int phys(int b)
 {
+    if constexpr (Wave<N>::OK)
+        return (b % Wave<N>::W) * 256 + b / Wave<N>::W;
+    else
+        return b;
+}
+
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// Four in-wave radix-4 Stockham passes over the 256-point regions reg[p]
+// (lane l holds points l + 64r).  If FROM_REGS the first pass takes v[p][r]
+// instead of reading LDS.  If TO_REGS (inverse only) the last pass leaves
+// its outputs (points l + 64r) in v instead of LDS.
+template <int N, bool INV, bool FROM_REGS, bool TO_REGS>
+__device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *reg0, cplx *reg1, int l,
+                                            const cplx *ztab)
+{
+    cplx *reg[2] = {reg0, reg1};
 #pragma unroll
-    for (int b2 = 0; b2 < 2; b2++) {
-        const cplx *blk = bsk_g + ((size_t)(a * 2 * L + 2 * lev + b2) * m + p) * M;
-        dst[b2][0] = blk[r2];
-        dst[b2][1] = blk[r2 + 1];
+    for (int Ns = 1; Ns < 256; Ns *= 4) {
+        cplx x[2][4];
+        if (FROM_REGS && Ns == 1) {
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    x[p][r] = v[p][r];
+        } else {
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    x[p][r] = reg[p][l + 64 * r];
+        }
+        const int k = l & (Ns - 1);
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+            r4_twiddle<N>(x[p], k, Ns, INV, ztab);
+        if (TO_REGS && Ns == 64) {
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    v[p][r] = x[p][r];
+            return;
+        }
+        wave_sync();
+        const int idxD = (l / Ns) * 4 * Ns + k;
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                reg[p][idxD + r * Ns] = x[p][r];
+        wave_sync();
     }
 }
 
-// W[bc2][c][.] += BSK(p) * rotation root of ciphertext c, for bins r and
-// M-1-r (the latter in conjugate form, see tfhe_device.h).
-template <int N, int CT>
-__device__ __forceinline__ void mac_pattern(cplx (&W)[2][CT][2], const cplx (&bk)[2][2],
+// Forward transform of two polynomials (v[p][r] = point t + r*T) into
+// buf[p*M + phys(bin)].  Called by all threads of the workgroup.
+template <int N>
+__device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
+{
+    constexpr int M = N / 2, W = Wave<N>::W;
+    const int wv = t >> 6, l = t & 63;
+    if constexpr (W == 1) {
+        __syncthreads();
+        wave_passes<N, false, true, false>(v, buf, buf + M, l, ztab);
+    } else {
+        // cross-wave DIF stage: z_k(n) = (sum_r x_{n + r M/W} w_W^{rk}) * w_M^{nk}
+        cplx z[2][4];
+        int pos[4];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            if constexpr (W == 4) {
+                cplx y[4] = {v[p][0], v[p][1], v[p][2], v[p][3]};
+                dft4(y, false);                                 // y_k, n = t
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    z[p][k] = k ? cmul(y[k], zeta_pow<N>(ztab, 4 * t * k)) : y[k];
+                    pos[k] = k * 256 + t;
+                }
+            } else {   // W == 2: points t, t+128 | t+256, t+384
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int n = t + 128 * h;
+                    z[p][2 * h] = cadd(v[p][h], v[p][h + 2]);                          // k = 0
+                    z[p][2 * h + 1] = cmul(csub(v[p][h], v[p][h + 2]), zeta_pow<N>(ztab, 4 * n));
+                    pos[2 * h] = n;
+                    pos[2 * h + 1] = 256 + n;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                buf[p * M + pos[k]] = z[p][k];
+        __syncthreads();
+        wave_passes<N, false, false, false>(v, buf + wv * 256, buf + M + wv * 256, l, ztab);
+    }
+    __syncthreads();
+}
+
+// Inverse (unnormalised) transform of buf[p*M + phys(bin)]; the result lands
+// in registers, v[p][r] = point t + r*T.  Called by all threads.
+template <int N>
+__device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
+{
+    constexpr int M = N / 2, W = Wave<N>::W;
+    const int wv = t >> 6, l = t & 63;
+    if constexpr (W == 1) {
+        wave_passes<N, true, false, true>(v, buf, buf + M, l, ztab);
+    } else {
+        wave_passes<N, true, false, false>(v, buf + wv * 256, buf + M + wv * 256, l, ztab);
+        __syncthreads();
+        // cross-wave DIT stage: x_{n + r M/W} = sum_k (u_k(n) w_M^{-nk}) w_W^{-rk}
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            if constexpr (W == 4) {
+                cplx y[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const cplx u = buf[p * M + k * 256 + t];
+                    y[k] = k ? cmul(u, zeta_pow<N>(ztab, -4 * t * k)) : u;
+                }
+                dft4(y, true);
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    v[p][r] = y[r];
+            } else {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int n = t + 128 * h;
+                    const cplx u0 = buf[p * M + n];
+                    const cplx u1 = cmul(buf[p * M + 256 + n], zeta_pow<N>(ztab, -4 * n));
+                    v[p][h] = cadd(u0, u1);
+                    v[p][h + 2] = csub(u0, u1);
+                }
+            }
+        }
+    }
+}
+
+// BSK values of one bin pair for pattern p of gadget level lev, output rows
+// a0 .. a0+NA-1: [a][bc2][0/1] = F-bins 2r, 2r+1 of BSK[g, a0+a, 2*lev+bc2, p].
+// The block pointer is uniform across the workgroup and the lane offset (2r)
+// is 32-bit.
+template <int L, int NA>
+__device__ __forceinline__ void load_pair(cplx (&dst)[NA][2][2], const cplx *__restrict__ bsk_g,
+                                          int r2, int a0, int lev, int p, int m, int M)
+{
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; b2++) {
+            const cplx *blk = bsk_g + ((size_t)((a0 + a) * 2 * L + 2 * lev + b2) * m + p) * M;
+            dst[a][b2][0] = blk[r2];
+            dst[a][b2][1] = blk[r2 + 1];
+        }
+}
+
+// W[a][bc2][c][.] += BSK(p) * rotation root of ciphertext c, for bins r and
+// M-1-r (the latter in conjugate form, see tfhe_device.h).  Each root serves
+// the NA output rows held.
+template <int N, int CT, int NA>
+__device__ __forceinline__ void mac_pattern(cplx (&W)[NA][2][CT][2], const cplx (&bk)[NA][2][2],
                                             const int32_t *const (&idp)[CT], int p, int r,
                                             const cplx *ztab)
 {
 #pragma unroll
     for (int c = 0; c < CT; c++) {
         const int e = idp[c][p];
-        const cplx rot0 = zeta_pow<N>(ztab, e * (4 * r + 1));
-        // zeta^(e(4(M-1-r)+1)) = conj(zeta^(e(4r+1)) * zeta^(2e))
-        const cplx rot1 = cconj(cmul(rot0, zeta_pow<N>(ztab, 2 * e)));
+        const int e0 = e * (4 * r + 1);
+        const cplx rot0 = zeta_pow<N>(ztab, e0);
+        // zeta^(e(4(M-1-r)+1)) = zeta^(-e(4r+3)) = conj(zeta^(e(4r+1) + 2e))
+        const cplx rot1 = cconj(zeta_pow<N>(ztab, e0 + 2 * e));
 #pragma unroll
-        for (int b2 = 0; b2 < 2; b2++) {
-            W[b2][c][0] = cfma(bk[b2][0], rot0, W[b2][c][0]);
-            W[b2][c][1] = cfma_conj(bk[b2][1], rot1, W[b2][c][1]);
-        }
+        for (int a = 0; a < NA; a++)
+#pragma unroll
+            for (int b2 = 0; b2 < 2; b2++) {
+                W[a][b2][c][0] = cfma(bk[a][b2][0], rot0, W[a][b2][c][0]);
+                W[a][b2][c][1] = cfma_conj(bk[a][b2][1], rot1, W[a][b2][c][1]);
+            }
     }
 }
 
-template <int N, int L, int CT>
+// T32: log2_q <= 32, the accumulator is held as its class mod 2^32 in
+// 32-bit registers (digit extraction reads bits below log2_q only, so the
+// class is all it needs); otherwise as 64-bit integers.
+template <int N, int L, int CT, bool T32>
 __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__restrict__ ct_init,
                                                         int64_t ct_stride,
                                                         const cplx *__restrict__ bsk,
@@ -236,9 +408,10 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T, BC = 2 * L;
     constexpr int PPT = 2 / CT;                  // bin pairs per thread in the MAC
+    constexpr int NA = (CT == 1 && N <= 2048) ? 2 : 1;   // output rows sharing one root lookup
     static_assert(CT == 1 || CT == 2, "one or two ciphertexts per workgroup");
     __shared__ cplx buf[CT * 2 * M];             // per ciphertext: two polynomials
-    __shared__ cplx ztab[M];
+    __shared__ cplx ztab[N];
 
     const int grp = threadIdx.x / T;             // ciphertext slot of this thread (transforms)
     const int t = threadIdx.x - grp * T;         // thread index within the slot
@@ -252,20 +425,15 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
     fill_root_table<N>(ztab);
 
     // accumulator rows in registers: x[c][i] = ct[c*N + t + i*T]
-    int64_t x[2][8];
+    using acc_t = typename std::conditional<T32, uint32_t, uint64_t>::type;
+    acc_t x[2][8];
     const int64_t *src = ct_init + ct_of[grp] * ct_stride;
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
         for (int i = 0; i < 8; i++)
-            x[c][i] = src[c * N + t + i * T];
+            x[c][i] = (acc_t)src[c * N + t + i * T];
     __syncthreads();
-
-    // twist factors zeta^k for the thread's folded points k = t + r*T
-    cplx tw[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-        tw[r] = zeta_pow<N>(ztab, t + r * T);
 
     // decomposition offset (dec_fft.cuh:7-17)
     uint64_t off = 0;
@@ -276,8 +444,9 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
         if (rs >= 0)
             off += 1ULL << rs;
     }
-    const uint64_t dmask = (1ULL << log2_beta) - 1;
-    const int64_t half_beta = 1LL << (log2_beta - 1);
+    const acc_t offs = (acc_t)off;      // off < 2^log2_q
+    const acc_t dmask = (acc_t)((1ULL << log2_beta) - 1);
+    const int half_beta = 1 << (log2_beta - 1);
 
     const double q = ldexp(1.0, log2_q);
     const double inv_q = ldexp(1.0, -log2_q);
@@ -311,61 +480,73 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
             for (int c = 0; c < 2; c++) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int64_t lo = (int64_t)((((uint64_t)x[c][r] + off) >> sh) & dmask) - half_beta;
-                    const int64_t hi = (int64_t)((((uint64_t)x[c][r + 4] + off) >> sh) & dmask) - half_beta;
-                    v[c][r] = cmul({(double)lo, -(double)hi}, tw[r]);   // (lo - i*hi) * zeta^k
+                    const int lo = (int)(((acc_t)(x[c][r] + offs) >> sh) & dmask) - half_beta;
+                    const int hi = (int)(((acc_t)(x[c][r + 4] + offs) >> sh) & dmask) - half_beta;
+                    v[c][r] = cmul({(double)lo, -(double)hi}, zeta_pow<N>(ztab, t + r * T));   // (lo - i*hi) * zeta^k
                 }
             }
-            forward_pair<N>(v, mybuf, ztab, t);
+            if constexpr (Wave<N>::OK)
+                forward_wave<N>(v, mybuf, ztab, t);
+            else
+                forward_pair<N>(v, mybuf, ztab, t);
 
             // ---- external product, level lev (mul_bootstrap.cuh:42-67)
 #pragma unroll 1
             for (int pp = 0; pp < PPT; pp++) {
                 const int r = CT == 1 ? u + pp * T : u;      // r < M/2; partner M-1-r
 #pragma unroll 1
-                for (int a = 0; a < 2; a++) {
-                    cplx W[2][CT][2];
+                for (int a0 = 0; a0 < 2; a0 += NA) {
+                    cplx W[NA][2][CT][2];
 #pragma unroll
-                    for (int b2 = 0; b2 < 2; b2++)
+                    for (int a = 0; a < NA; a++)
 #pragma unroll
-                        for (int c = 0; c < CT; c++)
-                            W[b2][c][0] = W[b2][c][1] = {0.0, 0.0};
+                        for (int b2 = 0; b2 < 2; b2++)
+#pragma unroll
+                            for (int c = 0; c < CT; c++)
+                                W[a][b2][c][0] = W[a][b2][c][1] = {0.0, 0.0};
                     // ping-pong prefetch: the loads of pattern p+1 fly while p is consumed
-                    cplx bkA[2][2], bkB[2][2];
-                    load_pair<L>(bkA, bsk_g, 2 * r, a, lev, 0, m, M);
+                    cplx bkA[NA][2][2], bkB[NA][2][2];
+                    load_pair<L, NA>(bkA, bsk_g, 2 * r, a0, lev, 0, m, M);
 #pragma unroll 1
                     for (int p = 0; p < m; p += 2) {
-                        load_pair<L>(bkB, bsk_g, 2 * r, a, lev, p + 1 < m ? p + 1 : p, m, M);
-                        mac_pattern<N, CT>(W, bkA, idp, p, r, ztab);
+                        load_pair<L, NA>(bkB, bsk_g, 2 * r, a0, lev, p + 1 < m ? p + 1 : p, m, M);
+                        mac_pattern<N, CT, NA>(W, bkA, idp, p, r, ztab);
                         if (p + 1 < m) {
-                            load_pair<L>(bkA, bsk_g, 2 * r, a, lev, p + 2 < m ? p + 2 : p + 1, m, M);
-                            mac_pattern<N, CT>(W, bkB, idp, p + 1, r, ztab);
+                            load_pair<L, NA>(bkA, bsk_g, 2 * r, a0, lev, p + 2 < m ? p + 2 : p + 1, m, M);
+                            mac_pattern<N, CT, NA>(W, bkB, idp, p + 1, r, ztab);
                         }
                     }
-                    cplx acc[CT][2];
+                    cplx acc[NA][CT][2];
 #pragma unroll
                     for (int c = 0; c < CT; c++) {
-                        acc[c][0] = acc[c][1] = {0.0, 0.0};
+#pragma unroll
+                        for (int a = 0; a < NA; a++)
+                            acc[a][c][0] = acc[a][c][1] = {0.0, 0.0};
 #pragma unroll
                         for (int b2 = 0; b2 < 2; b2++) {
-                            const cplx d0 = buf[c * 2 * M + b2 * M + r];
-                            const cplx d1 = buf[c * 2 * M + b2 * M + M - 1 - r];
-                            acc[c][0] = cfma(W[b2][c][0], d0, acc[c][0]);
-                            acc[c][1] = cfma(W[b2][c][1], d1, acc[c][1]);
+                            const cplx d0 = buf[c * 2 * M + b2 * M + phys<N>(r)];
+                            const cplx d1 = buf[c * 2 * M + b2 * M + phys<N>(M - 1 - r)];
+#pragma unroll
+                            for (int a = 0; a < NA; a++) {
+                                acc[a][c][0] = cfma(W[a][b2][c][0], d0, acc[a][c][0]);
+                                acc[a][c][1] = cfma(W[a][b2][c][1], d1, acc[a][c][1]);
+                            }
                         }
                     }
-                    // (a, pp) are uniform loop counters: select the accumulator
+                    // (a0, pp) are uniform loop counters: select the accumulator
                     // with compile-time indices so it stays in registers
 #pragma unroll
-                    for (int aa = 0; aa < 2; aa++)
+                    for (int aa = 0; aa < 2; aa += NA)
 #pragma unroll
                         for (int qq = 0; qq < PPT; qq++)
-                            if (aa == a && qq == pp)
+                            if (aa == a0 && qq == pp)
 #pragma unroll
-                                for (int c = 0; c < CT; c++) {
-                                    prod[aa][qq][c][0] = cadd(prod[aa][qq][c][0], acc[c][0]);
-                                    prod[aa][qq][c][1] = cadd(prod[aa][qq][c][1], acc[c][1]);
-                                }
+                                for (int a = 0; a < NA; a++)
+#pragma unroll
+                                    for (int c = 0; c < CT; c++) {
+                                        prod[aa + a][qq][c][0] = cadd(prod[aa + a][qq][c][0], acc[a][c][0]);
+                                        prod[aa + a][qq][c][1] = cadd(prod[aa + a][qq][c][1], acc[a][c][1]);
+                                    }
                 }
             }
         }
@@ -379,41 +560,59 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
             for (int c = 0; c < CT; c++)
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
-                    buf[c * 2 * M + a * M + r] = prod[a][pp][c][0];
-                    buf[c * 2 * M + a * M + M - 1 - r] = prod[a][pp][c][1];
+                    buf[c * 2 * M + a * M + phys<N>(r)] = prod[a][pp][c][0];
+                    buf[c * 2 * M + a * M + phys<N>(M - 1 - r)] = prod[a][pp][c][1];
                 }
         }
         __syncthreads();
         cplx w[2][4];
-        inverse_pair<N>(w, mybuf, ztab, t);
+        if constexpr (Wave<N>::OK)
+            inverse_wave<N>(w, mybuf, ztab, t);
+        else
+            inverse_pair<N>(w, mybuf, ztab, t);
 #pragma unroll
         for (int a = 0; a < 2; a++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const cplx z = cmul(w[a][r], cconj(tw[r]));
+                const cplx z = cmul(w[a][r], zeta_pow<N>(ztab, -(t + r * T)));
                 double lo = z.re * inv_m;
                 double hi = -z.im * inv_m;
-                lo = lo - q * rint(lo * inv_q);
-                hi = hi - q * rint(hi * inv_q);
-                lo = rint(lo);
-                hi = rint(hi);
-                if (log2_q >= 64) {   // +2^63 is the class of -2^63
-                    lo = lo >= 0x1p63 ? -0x1p63 : lo;
-                    hi = hi >= 0x1p63 ? -0x1p63 : hi;
+                if constexpr (T32) {
+                    // rint(val) - q*round_half_up(rint(val)/q) lies in [-q/2, q/2)
+                    // and is the class of the reference's round(val - q*rint(val/q))
+                    // (the subtraction is exact for |val| < 2^52)
+                    lo = rint(lo);
+                    hi = rint(hi);
+                    lo = lo - q * floor(fma(lo, inv_q, 0.5));
+                    hi = hi - q * floor(fma(hi, inv_q, 0.5));
+                    x[a][r] = (acc_t)(int)lo;
+                    x[a][r + 4] = (acc_t)(int)hi;
+                } else {
+                    lo = lo - q * rint(lo * inv_q);
+                    hi = hi - q * rint(hi * inv_q);
+                    lo = rint(lo);
+                    hi = rint(hi);
+                    if (log2_q >= 64) {   // +2^63 is the class of -2^63
+                        lo = lo >= 0x1p63 ? -0x1p63 : lo;
+                        hi = hi >= 0x1p63 ? -0x1p63 : hi;
+                    }
+                    x[a][r] = (acc_t)(int64_t)lo;
+                    x[a][r + 4] = (acc_t)(int64_t)hi;
                 }
-                x[a][r] = (int64_t)lo;
-                x[a][r + 4] = (int64_t)hi;
             }
         }
     }
 
     if (first + grp < batch) {
         int64_t *dst = out + (first + grp) * 2 * N;
+        // T32: centred representative in [-2^(log2_q-1), 2^(log2_q-1))
+        const int sh_out = 64 - log2_q;
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int i = 0; i < 8; i++)
-                dst[c * N + t + i * T] = x[c][i];
+                dst[c * N + t + i * T] =
+                    T32 ? ((int64_t)((uint64_t)x[c][i] << sh_out) >> sh_out) : (int64_t)x[c][i];
     }
 }
 
@@ -423,9 +622,14 @@ static int launch_br(hipStream_t stream, const int64_t *ct, int64_t ct_stride, c
                      int log2_beta, int log2_q)
 {
     const int64_t blocks = (batch + CT - 1) / CT;
-    hipLaunchKernelGGL((br_kernel<N, L, CT>), dim3((unsigned)blocks), dim3(N / 8 * CT), 0, stream,
-                       ct, ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out, batch, n_iter,
-                       m, log2_beta, log2_q);
+    if (log2_q <= 32)
+        hipLaunchKernelGGL((br_kernel<N, L, CT, true>), dim3((unsigned)blocks), dim3(N / 8 * CT), 0,
+                           stream, ct, ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out,
+                           batch, n_iter, m, log2_beta, log2_q);
+    else
+        hipLaunchKernelGGL((br_kernel<N, L, CT, false>), dim3((unsigned)blocks), dim3(N / 8 * CT), 0,
+                           stream, ct, ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out,
+                           batch, n_iter, m, log2_beta, log2_q);
     return launch_status("blind rotation");
 }
 

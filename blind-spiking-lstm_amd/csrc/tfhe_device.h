@@ -51,32 +51,27 @@ __device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
 __device__ __forceinline__ cplx cmul_mi(cplx a) { return {a.im, -a.re}; }
 __device__ __forceinline__ cplx cmul_pi(cplx a) { return {-a.im, a.re}; }
 
-// Root table in LDS: ztab[r] = zeta^r = exp(-i*pi*r/N) for r in [0, N/2).
+// Root table in LDS: ztab[r] = zeta^r = exp(-i*pi*r/N) for r in [0, N).
 template <int N>
 __device__ __forceinline__ void fill_root_table(cplx *ztab)
 {
-    for (int r = threadIdx.x; r < N / 2; r += blockDim.x) {
+    for (int r = threadIdx.x; r < N; r += blockDim.x) {
         double s, c;
         sincospi((double)r / (double)N, &s, &c);
         ztab[r] = {c, -s};
     }
 }
 
-// zeta^e for any integer e: zeta^(N/2) = -i, so zeta^e = (-i)^q * ztab[e mod N/2]
-// with q = (e mod 2N) div (N/2).  One 16-byte LDS read.
+// zeta^e for any integer e: zeta^N = -1, so zeta^e = +-ztab[e mod N], the
+// sign given by bit log2(N) of e.  One 16-byte LDS read and a sign flip of
+// both halves (xor on the high words).
 template <int N>
 __device__ __forceinline__ cplx zeta_pow(const cplx *ztab, int e)
 {
-    e &= 2 * N - 1;
-    const int q = e / (N / 2);
-    const cplx z = ztab[e & (N / 2 - 1)];
-    double re = (q & 1) ? z.im : z.re;
-    double im = (q & 1) ? -z.re : z.im;
-    if (q & 2) {
-        re = -re;
-        im = -im;
-    }
-    return {re, im};
+    const cplx z = ztab[e & (N - 1)];
+    const uint32_t flip = ((uint32_t)e & N) ? 0x80000000u : 0u;
+    return {__hiloint2double((int)(__double2hiint(z.re) ^ flip), __double2loint(z.re)),
+            __hiloint2double((int)(__double2hiint(z.im) ^ flip), __double2loint(z.im))};
 }
 
 }  // namespace bslstm

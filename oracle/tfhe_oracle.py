@@ -314,16 +314,16 @@ def negacyclic_matrix(s) -> np.ndarray:
 
 
 def negacyclic_mul_binary(a, s) -> np.ndarray:
-    """Exact a(X)*s(X) mod X^N+1 for |s_i| <= 1 and |a_j| < 2^31, batched
-    over leading axes of a.
+    """Exact a(X)*s(X) mod X^N+1 for |s_i| <= 1 and N*max|a_j| < 2^53,
+    batched over leading axes of a.
 
     A float64 matrix product with the negacyclic matrix of s: every partial
-    sum is an integer below N*2^31 < 2^53, so the result is exact whatever
+    sum is an integer below N*max|a| < 2^53, so the result is exact whatever
     the summation order.
     """
     a = np.asarray(a, dtype=np.int64)
     N = a.shape[-1]
-    assert np.abs(a).max(initial=0) <= (1 << 31) and N <= (1 << 21)
+    assert float(np.abs(a).max(initial=0)) * N < 2.0 ** 53
     T = negacyclic_matrix(np.asarray(s, np.float64))
     out = a.reshape(-1, N).astype(np.float64) @ T
     return np.rint(out).astype(np.int64).reshape(a.shape)

@@ -74,6 +74,25 @@ def test_mul_bootstrap_vs_exact_int128(cuda_device, mult, N, L, log_beta, form):
     assert np.abs(d).max() <= BR_COEF_TOL
 
 
+@pytest.mark.parametrize("log_q,L,lb,tol", [(36, 2, 8, 64 * 16), (24, 2, 6, 64), (20, 1, 10, 64)])
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_mul_bootstrap_other_moduli(cuda_device, mult, log_q, L, lb, tol, form):
+    """log2 q > 32 runs the 64-bit accumulator, log2 q < 32 the 32-bit one;
+    the FP64 deviation from the exact oracle grows with q."""
+    case = make_case(N=256, n=12, L=L, log_beta=lb, collapse=3, B=5, seed=log_q, log_q=log_q,
+                     sigma_bsk=2.0 ** (log_q - 26), sigma_lwe=2.0 ** (log_q - 15))
+    idx = O.rotation_indices(case.a, log_q, 256, 3, 1)
+    ct = np.broadcast_to(case.lut, (5, 2, 256))
+    got = host(mult.mul_bootstrap(dev(ct), dev(case.bsk_fft_k), None, dev(idx), lb, log_q, L,
+                                  form=form))
+    exact = X.blind_rotate(case.lut, case.bsk_coef_k, idx, lb, log_q, L)
+    d = O.centered_mod(got - exact, log_q)
+    print(f"log_q={log_q}: max|gpu-exact|={np.abs(d).max()}")
+    assert np.abs(d).max() <= tol
+    half = 1 << (log_q - 1)
+    assert got.min() >= -half and got.max() < half
+
+
 @pytest.mark.parametrize("N,L,c", [(2048, 3, 3), (1024, 2, 2), (512, 4, 1), (256, 3, 4)])
 def test_plain_and_batched_forms_bit_identical(cuda_device, mult, N, L, c):
     """Odd batch (a tail workgroup with one ciphertext) and per-ciphertext
