@@ -197,7 +197,7 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const c
 // One radix-W stage crosses waves (a register butterfly plus one LDS
 // exchange); the remaining four radix-4 Stockham passes run inside a wave,
 // ordered by the wave's own in-order LDS traffic, with no workgroup barrier.
-// Bin b of the result sits at phys(b) = (b mod W)*256 + b/W.
+// Bin b of the result is element b/W of region b mod W (see region_slot).
 // ---------------------------------------------------------------------------
 template <int N>
 struct Wave {
@@ -205,11 +205,21 @@ struct Wave {
     static constexpr bool OK = (N >= 512 && N <= 2048);
 };
 
+// Position of element e of region k in LDS.  The XOR keeps every access
+// pattern of the transform (contiguous reads, the strided Stockham writes of
+// each pass) and the MAC's gathers across regions free of bank conflicts.
+template <int N>
+__device__ __forceinline__ int region_slot(int k, int e)
+{
+    constexpr int W = Wave<N>::W;
+    return k * 256 + (e ^ ((0xF5A0 >> (4 * ((e >> 4) & 3))) & 15) ^ (k * (16 / W)));
+}
+
 template <int N>
 __device__ __forceinline__ int phys(int b)
 {
     if constexpr (Wave<N>::OK)
-        return (b % Wave<N>::W) * 256 + b / Wave<N>::W;
+        return region_slot<N>(b % Wave<N>::W, b / Wave<N>::W);
     else
         return b;
 }
@@ -221,10 +231,11 @@ __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 // instead of reading LDS.  If TO_REGS (inverse only) the last pass leaves
 // its outputs (points l + 64r) in v instead of LDS.
 template <int N, bool INV, bool FROM_REGS, bool TO_REGS>
-__device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *reg0, cplx *reg1, int l,
+__device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *poly0, cplx *poly1, int rg, int l,
                                             const cplx *ztab)
 {
-    cplx *reg[2] = {reg0, reg1};
+    // element e of this wave's region rg of polynomial p: reg[p][region_slot(rg, e)]
+    cplx *reg[2] = {poly0, poly1};
 #pragma unroll
     for (int Ns = 1; Ns < 256; Ns *= 4) {
         cplx x[2][4];
@@ -239,7 +250,7 @@ __device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *reg0, cplx *r
             for (int p = 0; p < 2; p++)
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    x[p][r] = reg[p][l + 64 * r];
+                    x[p][r] = reg[p][region_slot<N>(rg, l + 64 * r)];
         }
         const int k = l & (Ns - 1);
 #pragma unroll
@@ -259,7 +270,7 @@ __device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *reg0, cplx *r
         for (int p = 0; p < 2; p++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
-                reg[p][idxD + r * Ns] = x[p][r];
+                reg[p][region_slot<N>(rg, idxD + r * Ns)] = x[p][r];
         wave_sync();
     }
 }
@@ -273,7 +284,7 @@ __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const c
     const int wv = t >> 6, l = t & 63;
     if constexpr (W == 1) {
         __syncthreads();
-        wave_passes<N, false, true, false>(v, buf, buf + M, l, ztab);
+        wave_passes<N, false, true, false>(v, buf, buf + M, 0, l, ztab);
     } else {
         // cross-wave DIF stage: z_k(n) = (sum_r x_{n + r M/W} w_W^{rk}) * w_M^{nk}
         cplx z[2][4];
@@ -286,7 +297,7 @@ __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const c
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     z[p][k] = k ? cmul(y[k], zeta_pow<N>(ztab, 4 * t * k)) : y[k];
-                    pos[k] = k * 256 + t;
+                    pos[k] = region_slot<N>(k, t);
                 }
             } else {   // W == 2: points t, t+128 | t+256, t+384
 #pragma unroll
@@ -294,8 +305,8 @@ __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const c
                     const int n = t + 128 * h;
                     z[p][2 * h] = cadd(v[p][h], v[p][h + 2]);                          // k = 0
                     z[p][2 * h + 1] = cmul(csub(v[p][h], v[p][h + 2]), zeta_pow<N>(ztab, 4 * n));
-                    pos[2 * h] = n;
-                    pos[2 * h + 1] = 256 + n;
+                    pos[2 * h] = region_slot<N>(0, n);
+                    pos[2 * h + 1] = region_slot<N>(1, n);
                 }
             }
         }
@@ -306,7 +317,7 @@ __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const c
             for (int k = 0; k < 4; k++)
                 buf[p * M + pos[k]] = z[p][k];
         __syncthreads();
-        wave_passes<N, false, false, false>(v, buf + wv * 256, buf + M + wv * 256, l, ztab);
+        wave_passes<N, false, false, false>(v, buf, buf + M, wv, l, ztab);
     }
     __syncthreads();
 }
@@ -319,9 +330,9 @@ __device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const c
     constexpr int M = N / 2, W = Wave<N>::W;
     const int wv = t >> 6, l = t & 63;
     if constexpr (W == 1) {
-        wave_passes<N, true, false, true>(v, buf, buf + M, l, ztab);
+        wave_passes<N, true, false, true>(v, buf, buf + M, 0, l, ztab);
     } else {
-        wave_passes<N, true, false, false>(v, buf + wv * 256, buf + M + wv * 256, l, ztab);
+        wave_passes<N, true, false, false>(v, buf, buf + M, wv, l, ztab);
         __syncthreads();
         // cross-wave DIT stage: x_{n + r M/W} = sum_k (u_k(n) w_M^{-nk}) w_W^{-rk}
 #pragma unroll
@@ -330,7 +341,7 @@ __device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const c
                 cplx y[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const cplx u = buf[p * M + k * 256 + t];
+                    const cplx u = buf[p * M + region_slot<N>(k, t)];
                     y[k] = k ? cmul(u, zeta_pow<N>(ztab, -4 * t * k)) : u;
                 }
                 dft4(y, true);
@@ -341,8 +352,8 @@ __device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const c
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const int n = t + 128 * h;
-                    const cplx u0 = buf[p * M + n];
-                    const cplx u1 = cmul(buf[p * M + 256 + n], zeta_pow<N>(ztab, -4 * n));
+                    const cplx u0 = buf[p * M + region_slot<N>(0, n)];
+                    const cplx u1 = cmul(buf[p * M + region_slot<N>(1, n)], zeta_pow<N>(ztab, -4 * n));
                     v[p][h] = cadd(u0, u1);
                     v[p][h + 2] = csub(u0, u1);
                 }

@@ -51,24 +51,40 @@ __device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
 __device__ __forceinline__ cplx cmul_mi(cplx a) { return {a.im, -a.re}; }
 __device__ __forceinline__ cplx cmul_pi(cplx a) { return {-a.im, a.re}; }
 
-// Root table in LDS: ztab[r] = zeta^r = exp(-i*pi*r/N) for r in [0, N).
+// Root table in LDS: zeta^r = exp(-i*pi*r/N) for r in [0, N), stored at
+// root_slot(r).  The slot XORs the low four index bits with a fold of the
+// higher ones: lanes reading roots whose exponents step by a power of two
+// (rotation roots e*(4r+1), transform twiddles) then spread over the LDS
+// bank groups instead of piling onto one.
+template <int N>
+__device__ __forceinline__ int root_slot(int r)
+{
+    int h = r >> 4, f = 0;
+#pragma unroll
+    for (int b = 4; b < 13; b += 4) {
+        f ^= h & 15;
+        h >>= 4;
+    }
+    return r ^ f;
+}
+
 template <int N>
 __device__ __forceinline__ void fill_root_table(cplx *ztab)
 {
     for (int r = threadIdx.x; r < N; r += blockDim.x) {
         double s, c;
         sincospi((double)r / (double)N, &s, &c);
-        ztab[r] = {c, -s};
+        ztab[root_slot<N>(r)] = {c, -s};
     }
 }
 
-// zeta^e for any integer e: zeta^N = -1, so zeta^e = +-ztab[e mod N], the
+// zeta^e for any integer e: zeta^N = -1, so zeta^e = +-zeta^(e mod N), the
 // sign given by bit log2(N) of e.  One 16-byte LDS read and a sign flip of
 // both halves (xor on the high words).
 template <int N>
 __device__ __forceinline__ cplx zeta_pow(const cplx *ztab, int e)
 {
-    const cplx z = ztab[e & (N - 1)];
+    const cplx z = ztab[root_slot<N>(e & (N - 1))];
     const uint32_t flip = ((uint32_t)e & N) ? 0x80000000u : 0u;
     return {__hiloint2double((int)(__double2hiint(z.re) ^ flip), __double2loint(z.re)),
             __hiloint2double((int)(__double2hiint(z.im) ^ flip), __double2loint(z.im))};
