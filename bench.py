@@ -29,6 +29,12 @@ CFG = dict(B=16384, n=750, N=2048, L=3, log_beta=7, collapse=3, n_lut=4, msg_bit
 
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per launch of the blind-rotation kernel at this workload (16384
+# ciphertexts), from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), the
+# scales read off a 1 GiB torch.neg calibration kernel in the same runs
+# (profiles/br_kernel_2048_3_pbs16384.md).  None for other batch sizes.
+HBM_TRAFFIC_BYTES = {"batched": 101.53e9 + 0.604e9, "plain": 4.001e12 + 0.570e9}
+
 
 def fp64_flops_per_bootstrap(n, N, L, c):
     """Algorithmic FP64 work of one collapsed blind rotation in the folded
@@ -169,7 +175,7 @@ def run_gpu(args):
                    "n_lut": cfg["n_lut"], "torus": cfg["log_q"], "form": main_form + "_fp64"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": (HBM_TRAFFIC_BYTES[main_form] if B == CFG["B"] else None),
                      "kernel": f"br_kernel<2048,3,{1 if main_form == 'plain' else 2}>",
                      "kernel_avg_ms": round(br_avg_ms, 3)},
         "decrypt_ok_sample": ok_frac,
