@@ -34,333 +34,9 @@
 #include <type_traits>
 
 #include "capi_internal.h"
-#include "tfhe_device.h"
+#include "fold_fft.h"
 
 namespace bslstm {
-
-template <int N>
-struct Fold {
-    static constexpr int M = N / 2;      // complex points per polynomial
-    static constexpr int T = N / 8;      // threads per ciphertext
-    static constexpr int LOG2M = __builtin_ctz(M);
-    static constexpr bool ODD = (LOG2M & 1) != 0;   // one radix-2 pass needed
-};
-
-__device__ __forceinline__ void dft4(cplx x[4], bool inv)
-{
-    cplx t0 = cadd(x[0], x[2]);
-    cplx t1 = csub(x[0], x[2]);
-    cplx t2 = cadd(x[1], x[3]);
-    cplx t3 = csub(x[1], x[3]);
-    x[0] = cadd(t0, t2);
-    x[2] = csub(t0, t2);
-    cplx r = inv ? cmul_pi(t3) : cmul_mi(t3);
-    x[1] = cadd(t1, r);
-    x[3] = csub(t1, r);
-}
-
-// Twiddle then radix-4 butterfly for Stockham span Ns (inputs j + r*M/4).
-template <int N>
-__device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, const cplx *ztab)
-{
-    if (Ns > 1) {
-        const int base = (Fold<N>::M / Ns) * k;     // zeta exponent of w^k, w = exp(-2 pi i/(4 Ns))
-#pragma unroll
-        for (int r = 1; r < 4; r++) {
-            cplx w = zeta_pow<N>(ztab, inv ? -r * base : r * base);
-            x[r] = cmul(x[r], w);
-        }
-    }
-    dft4(x, inv);
-}
-
-// Forward transform of two polynomials held in registers (v[p][r] = point
-// t + r*T, t = thread index within the ciphertext's group); result left in
-// buf[p*M + bin] in natural bin order.  All threads of the workgroup call it
-// (it contains workgroup barriers).
-template <int N>
-__device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
-{
-    constexpr int M = Fold<N>::M, T = Fold<N>::T;
-    int Ns;
-    if constexpr (Fold<N>::ODD) {
-        // radix-2, span 1: butterflies j = t (points t, t+2T) and j = t+T (t+T, t+3T)
-        cplx o[2][4];
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            o[p][0] = cadd(v[p][0], v[p][2]);
-            o[p][1] = csub(v[p][0], v[p][2]);
-            o[p][2] = cadd(v[p][1], v[p][3]);
-            o[p][3] = csub(v[p][1], v[p][3]);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            buf[p * M + 2 * t] = o[p][0];
-            buf[p * M + 2 * t + 1] = o[p][1];
-            buf[p * M + 2 * (t + T)] = o[p][2];
-            buf[p * M + 2 * (t + T) + 1] = o[p][3];
-        }
-        __syncthreads();
-        Ns = 2;
-    } else {
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-            dft4(v[p], false);
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                buf[p * M + 4 * t + r] = v[p][r];
-        __syncthreads();
-        Ns = 4;
-    }
-    for (; Ns < M; Ns *= 4) {
-        cplx x[2][4];
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                x[p][r] = buf[p * M + t + r * T];
-        const int k = t & (Ns - 1);
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-            r4_twiddle<N>(x[p], k, Ns, false, ztab);
-        __syncthreads();
-        const int idxD = (t / Ns) * 4 * Ns + k;
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                buf[p * M + idxD + r * Ns] = x[p][r];
-        __syncthreads();
-    }
-}
-
-// Inverse (unnormalised) transform of buf[p*M + bin]; the result lands in
-// registers, v[p][r] = point t + r*T.  Called by all threads of the workgroup.
-template <int N>
-__device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
-{
-    constexpr int M = Fold<N>::M, T = Fold<N>::T;
-    constexpr int R4_END = Fold<N>::ODD ? M / 2 : M;   // radix-4 passes cover spans up to R4_END/4
-    for (int Ns = 1; Ns < R4_END; Ns *= 4) {
-        cplx x[2][4];
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                x[p][r] = buf[p * M + t + r * T];
-        const int k = t & (Ns - 1);
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-            r4_twiddle<N>(x[p], k, Ns, true, ztab);
-        if (!Fold<N>::ODD && Ns * 4 == M) {
-            // last pass: idxD == t, outputs at t + r*T
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    v[p][r] = x[p][r];
-            return;
-        }
-        __syncthreads();
-        const int idxD = (t / Ns) * 4 * Ns + k;
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                buf[p * M + idxD + r * Ns] = x[p][r];
-        __syncthreads();
-    }
-    if constexpr (Fold<N>::ODD) {
-        // radix-2, span M/2: butterflies j = t and j = t+T, inputs j and j+M/2
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int j = t + h * T;
-                cplx x0 = buf[p * M + j];
-                cplx x1 = buf[p * M + j + M / 2];
-                x1 = cmul(x1, zeta_pow<N>(ztab, -4 * j));   // conj(w^j), w = exp(-2 pi i/M)
-                v[p][h] = cadd(x0, x1);                    // point j
-                v[p][h + 2] = csub(x0, x1);                // point j + M/2
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Wave-local transform for N in {512, 1024, 2048}: W = N/512 waves per
-// ciphertext, each wave owning a 256-point region of the N/2-point transform.
-// One radix-W stage crosses waves (a register butterfly plus one LDS
-// exchange); the remaining four radix-4 Stockham passes run inside a wave,
-// ordered by the wave's own in-order LDS traffic, with no workgroup barrier.
-// Bin b of the result is element b/W of region b mod W (see region_slot).
-// ---------------------------------------------------------------------------
-template <int N>
-struct Wave {
-    static constexpr int W = N / 512;
-    static constexpr bool OK = (N >= 512 && N <= 2048);
-};
-
-// Position of element e of region k in LDS.  The XOR keeps every access
-// pattern of the transform (contiguous reads, the strided Stockham writes of
-// each pass) and the MAC's gathers across regions free of bank conflicts.
-template <int N>
-__device__ __forceinline__ int region_slot(int k, int e)
-{
-    constexpr int W = Wave<N>::W;
-    return k * 256 + (e ^ ((0xF5A0 >> (4 * ((e >> 4) & 3))) & 15) ^ (k * (16 / W)));
-}
-
-template <int N>
-__device__ __forceinline__ int phys(int b)
-{
-    if constexpr (Wave<N>::OK)
-        return region_slot<N>(b % Wave<N>::W, b / Wave<N>::W);
-    else
-        return b;
-}
-
-__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
-
-// Four in-wave radix-4 Stockham passes over the 256-point regions reg[p]
-// (lane l holds points l + 64r).  If FROM_REGS the first pass takes v[p][r]
-// instead of reading LDS.  If TO_REGS (inverse only) the last pass leaves
-// its outputs (points l + 64r) in v instead of LDS.
-template <int N, bool INV, bool FROM_REGS, bool TO_REGS>
-__device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *poly0, cplx *poly1, int rg, int l,
-                                            const cplx *ztab)
-{
-    // element e of this wave's region rg of polynomial p: reg[p][region_slot(rg, e)]
-    cplx *reg[2] = {poly0, poly1};
-#pragma unroll
-    for (int Ns = 1; Ns < 256; Ns *= 4) {
-        cplx x[2][4];
-        if (FROM_REGS && Ns == 1) {
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    x[p][r] = v[p][r];
-        } else {
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    x[p][r] = reg[p][region_slot<N>(rg, l + 64 * r)];
-        }
-        const int k = l & (Ns - 1);
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-            r4_twiddle<N>(x[p], k, Ns, INV, ztab);
-        if (TO_REGS && Ns == 64) {
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    v[p][r] = x[p][r];
-            return;
-        }
-        wave_sync();
-        const int idxD = (l / Ns) * 4 * Ns + k;
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                reg[p][region_slot<N>(rg, idxD + r * Ns)] = x[p][r];
-        wave_sync();
-    }
-}
-
-// Forward transform of two polynomials (v[p][r] = point t + r*T) into
-// buf[p*M + phys(bin)].  Called by all threads of the workgroup.
-template <int N>
-__device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
-{
-    constexpr int M = N / 2, W = Wave<N>::W;
-    const int wv = t >> 6, l = t & 63;
-    if constexpr (W == 1) {
-        __syncthreads();
-        wave_passes<N, false, true, false>(v, buf, buf + M, 0, l, ztab);
-    } else {
-        // cross-wave DIF stage: z_k(n) = (sum_r x_{n + r M/W} w_W^{rk}) * w_M^{nk}
-        cplx z[2][4];
-        int pos[4];
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            if constexpr (W == 4) {
-                cplx y[4] = {v[p][0], v[p][1], v[p][2], v[p][3]};
-                dft4(y, false);                                 // y_k, n = t
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    z[p][k] = k ? cmul(y[k], zeta_pow<N>(ztab, 4 * t * k)) : y[k];
-                    pos[k] = region_slot<N>(k, t);
-                }
-            } else {   // W == 2: points t, t+128 | t+256, t+384
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int n = t + 128 * h;
-                    z[p][2 * h] = cadd(v[p][h], v[p][h + 2]);                          // k = 0
-                    z[p][2 * h + 1] = cmul(csub(v[p][h], v[p][h + 2]), zeta_pow<N>(ztab, 4 * n));
-                    pos[2 * h] = region_slot<N>(0, n);
-                    pos[2 * h + 1] = region_slot<N>(1, n);
-                }
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                buf[p * M + pos[k]] = z[p][k];
-        __syncthreads();
-        wave_passes<N, false, false, false>(v, buf, buf + M, wv, l, ztab);
-    }
-    __syncthreads();
-}
-
-// Inverse (unnormalised) transform of buf[p*M + phys(bin)]; the result lands
-// in registers, v[p][r] = point t + r*T.  Called by all threads.
-template <int N>
-__device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
-{
-    constexpr int M = N / 2, W = Wave<N>::W;
-    const int wv = t >> 6, l = t & 63;
-    if constexpr (W == 1) {
-        wave_passes<N, true, false, true>(v, buf, buf + M, 0, l, ztab);
-    } else {
-        wave_passes<N, true, false, false>(v, buf, buf + M, wv, l, ztab);
-        __syncthreads();
-        // cross-wave DIT stage: x_{n + r M/W} = sum_k (u_k(n) w_M^{-nk}) w_W^{-rk}
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            if constexpr (W == 4) {
-                cplx y[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const cplx u = buf[p * M + region_slot<N>(k, t)];
-                    y[k] = k ? cmul(u, zeta_pow<N>(ztab, -4 * t * k)) : u;
-                }
-                dft4(y, true);
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    v[p][r] = y[r];
-            } else {
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int n = t + 128 * h;
-                    const cplx u0 = buf[p * M + region_slot<N>(0, n)];
-                    const cplx u1 = cmul(buf[p * M + region_slot<N>(1, n)], zeta_pow<N>(ztab, -4 * n));
-                    v[p][h] = cadd(u0, u1);
-                    v[p][h + 2] = csub(u0, u1);
-                }
-            }
-        }
-    }
-}
 
 // BSK values of one bin pair for pattern p of gadget level lev, output rows
 // a0 .. a0+NA-1: [a][bc2][0/1] = F-bins 2r, 2r+1 of BSK[g, a0+a, 2*lev+bc2, p].

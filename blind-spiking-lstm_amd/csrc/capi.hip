@@ -23,4 +23,4 @@ void clear_error() { g_last_error[0] = '\0'; }
 
 extern "C" const char *bslstm_last_error(void) { return bslstm::g_last_error; }
 
-extern "C" int bslstm_abi_version(void) { return 1; }
+extern "C" int bslstm_abi_version(void) { return 2; }

@@ -1,0 +1,850 @@
+// gfx950 kernels of the encrypted spiking-LSTM timestep around the
+// programmable bootstrap (nn_tfhe/Spike_LSTM.py:103-184 of the reference):
+//
+//   linear_extract_kernel   homomorphic Linear: plaintext weight polynomial x
+//                           RLWE, coefficient-0 sample extraction, bias
+//                           (Spike_LSTM.py:46-61), exact
+//   trace_pack_kernel       LWE -> RLWE by modulus switch, trivial embedding
+//                           and the log2(N)-step automorphism/key-switch
+//                           trace (text_jax.py:268-345)
+//   lut_product_kernel      plaintext LUT x RLWE products summed over the
+//                           many-LUT slots (activation.py:103-106,
+//                           Spike_LSTM.py:129-132)
+//   pack_transpose_kernel,  packing of up to N LWEs into one RLWE with the
+//   packing_mac_kernel,     packing key switching key (text_jax.py:211-239)
+//   packing_finalize_kernel
+//   poly_fft_kernel         negacyclic transform of integer polynomials into
+//                           the reference's bin order (keys, LUTs)
+//   torus_axpy_kernel,      sums of ciphertexts and X^e rotations
+//   rotate_poly_kernel      (text_jax.py:27-43, 348-362)
+//
+// Torus values are int64 classes mod q = 2^log2_q, log2_q <= 64, stored as
+// centred representatives; integer sums wrap mod 2^64, which is exact mod q.
+// Products of a torus polynomial with a gadget digit or a LUT polynomial
+// go through the folded FP64 transform of fold_fft.h, as the reference's
+// jax_fourier products do; everything else is exact integer arithmetic.
+#include <algorithm>
+
+#include "capi_internal.h"
+#include "fold_fft.h"
+
+namespace bslstm {
+
+// centred representative mod 2^lq of a 64-bit class
+__device__ __forceinline__ int64_t tmod(uint64_t x, int lq)
+{
+    const int s = 64 - lq;
+    return (int64_t)(x << s) >> s;
+}
+
+// round_half_even(r / 2^s) for s in [1, 63], without overflow
+__device__ __forceinline__ int64_t rhe_shift(int64_t r, int s)
+{
+    const int64_t fl = r >> s;
+    const uint64_t rem = (uint64_t)r & ((1ULL << s) - 1);
+    const uint64_t half = 1ULL << (s - 1);
+    return fl + (int64_t)((rem > half) | ((rem == half) & ((uint64_t)fl & 1)));
+}
+
+// Next signed digit of the reference's float decomposition
+// (polynomial_jax.py:90-111: x = gamma/q; d = round(x*beta); x = x*beta - d)
+// restated exactly on the centred integer residual r: d = round_half_even(
+// r / 2^s), r -= d*2^s, with s = log2_q - level*log2_beta (levels from 1).
+__device__ __forceinline__ int64_t next_digit(int64_t &r, int s)
+{
+    if (s <= 0) {
+        const int64_t d = r;
+        r = 0;
+        return d;
+    }
+    const int64_t d = rhe_shift(r, s);
+    r = (int64_t)((uint64_t)r - ((uint64_t)d << s));
+    return d;
+}
+
+// Torus class of a rounded FP64 transform output (jax_inversefourier's
+// round, then centred_mod q).
+__device__ __forceinline__ int64_t f64_to_torus(double v, double q, double inv_q, int lq)
+{
+    v = rint(v - q * rint(v * inv_q));
+    if (lq >= 64)
+        return v >= 0x1p63 ? INT64_MIN : (int64_t)v;
+    return tmod((uint64_t)(int64_t)v, lq);
+}
+
+template <int N>
+__device__ __forceinline__ void forward_any(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
+{
+    if constexpr (Wave<N>::OK)
+        forward_wave<N>(v, buf, ztab, t);
+    else
+        forward_pair<N>(v, buf, ztab, t);
+}
+
+template <int N>
+__device__ __forceinline__ void inverse_any(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
+{
+    if constexpr (Wave<N>::OK)
+        inverse_wave<N>(v, buf, ztab, t);
+    else
+        inverse_pair<N>(v, buf, ztab, t);
+}
+
+// fold + twist of two real polynomials given by their coefficients
+// lo[r] = p_{t+rT}, hi[r] = p_{t+rT+N/2}: u_k = (p_k - i p_{k+N/2}) zeta^k
+template <int N>
+__device__ __forceinline__ cplx fold_point(double lo, double hi, int t, int r, const cplx *ztab)
+{
+    return cmul({lo, -hi}, zeta_pow<N>(ztab, t + r * Fold<N>::T));
+}
+
+// inverse fold: coefficients k = t+rT (lo) and k+N/2 (hi) of an unnormalised
+// inverse transform output w
+template <int N>
+__device__ __forceinline__ void unfold_point(cplx w, int t, int r, const cplx *ztab, double &lo,
+                                             double &hi)
+{
+    const cplx z = cmul(w, zeta_pow<N>(ztab, -(t + r * Fold<N>::T)));
+    constexpr double inv_m = 1.0 / (double)Fold<N>::M;
+    lo = z.re * inv_m;
+    hi = -z.im * inv_m;
+}
+
+// ---------------------------------------------------------------------------
+// Homomorphic Linear (Spike_LSTM.py:27-61)
+//
+// For output row o with weight polynomial W_o (plaintext integers) and the
+// input RLWE (A, B): mask_j = coefficient -j of W_o*A in Z[X]/(X^N+1), i.e.
+//   mask_j = sum_i W_o[i] * E[(2N - j - i) mod 2N],  E = (A, -A) (the
+//   negacyclic extension), body = sum_i W_o[i] * E_B[(2N - i) mod 2N] + bias_o.
+// This is sample_extract(W_o*ct, 0) (text_jax.py:191-208) of the exact
+// product; the reference forms the product with a rounded FP64 FFT
+// (fft_product_twice, polynomial_jax.py:45-54).  Only the weight positions
+// listed in `support` (the nonzero columns, uniform across the launch) are
+// visited.
+// ---------------------------------------------------------------------------
+constexpr int LIN_THREADS = 256;
+constexpr int LIN_OT = 8;   // output rows per workgroup
+
+__global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
+    const int64_t *__restrict__ ct, int64_t ct_stride, const int64_t *__restrict__ w,
+    const int32_t *__restrict__ support, int K, const int64_t *__restrict__ bias, int out_dim, int N,
+    int lq, int64_t *__restrict__ masks, int64_t *__restrict__ bodies)
+{
+    extern __shared__ int64_t ext[];   // 2N: E[v] = A_v (v < N), -A_{v-N}
+    const int b = blockIdx.z;
+    const int o0 = blockIdx.y * LIN_OT;
+    const int64_t *A = ct + (size_t)b * ct_stride;
+    const int64_t *Bp = A + N;
+    for (int k = threadIdx.x; k < N; k += LIN_THREADS) {
+        const int64_t a = A[k];
+        ext[k] = a;
+        ext[N + k] = (int64_t)(0ULL - (uint64_t)a);
+    }
+    __syncthreads();
+
+    const int j = blockIdx.x * LIN_THREADS + threadIdx.x;
+    if (j < N) {
+        uint64_t acc[LIN_OT];
+#pragma unroll
+        for (int oo = 0; oo < LIN_OT; oo++)
+            acc[oo] = 0;
+        for (int s = 0; s < K; s++) {
+            const int i = support[s];
+            const uint64_t av = (uint64_t)ext[(2 * N - j - i) & (2 * N - 1)];
+#pragma unroll
+            for (int oo = 0; oo < LIN_OT; oo++)
+                if (o0 + oo < out_dim)
+                    acc[oo] += (uint64_t)w[(size_t)(o0 + oo) * N + i] * av;
+        }
+#pragma unroll
+        for (int oo = 0; oo < LIN_OT; oo++)
+            if (o0 + oo < out_dim)
+                masks[((size_t)b * out_dim + o0 + oo) * N + j] = tmod(acc[oo], lq);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < LIN_OT && o0 + (int)threadIdx.x < out_dim) {
+        const int o = o0 + threadIdx.x;
+        uint64_t acc = bias ? (uint64_t)bias[o] : 0ULL;
+        for (int s = 0; s < K; s++) {
+            const int i = support[s];
+            // coefficient 0 of W*B: W_0 B_0 - sum_{i>0} W_i B_{N-i}
+            const uint64_t bv = i == 0 ? (uint64_t)Bp[0] : 0ULL - (uint64_t)Bp[N - i];
+            acc += (uint64_t)w[(size_t)o * N + i] * bv;
+        }
+        bodies[(size_t)b * out_dim + o] = tmod(acc, lq);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Trace packing of one LWE into an RLWE (pack_lwe_to_rlwe, text_jax.py:314-345)
+//
+//   (a, b) <- round(x / N)                      modulus_switch to q/N
+//   A = a_0 - sum_{i>0} a_{N-i} X^i, B = b      trivial_embedding (:268-287)
+//   for step s < log2 N, k = 2^(log2 N - s) + 1:
+//     (A', B') = (A(X^k), B(X^k))               apply_automorphism
+//     (A_ks, B_ks) = sum_lev dec_lev(A') * G[s, lev]   (FFT-domain gadget
+//                                                product with the Galois key)
+//     A <- A + A_ks,  B <- B + B' + B_ks         (:290-311, :335-343)
+// One workgroup of N/8 threads per LWE; the RLWE stays in registers
+// (coefficients t + i*N/8), the automorphism permutes it through LDS.
+// gkey is (log2 N, l, 2, N/2) complex in the reference's bin order.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__restrict__ lwe_a,
+                                                           const int64_t *__restrict__ lwe_b,
+                                                           const cplx *__restrict__ gkey, int l,
+                                                           int lb, int lq,
+                                                           int64_t *__restrict__ out)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T, LOGN = __builtin_ctz(N);
+    __shared__ cplx buf[2 * M];
+    __shared__ cplx ztab[N];
+    int64_t *xb = reinterpret_cast<int64_t *>(buf);   // 2N int64 for the permutation
+    const int t = threadIdx.x;
+    const size_t wid = blockIdx.x;
+    fill_root_table<N>(ztab);
+
+    uint64_t A[8], B[8];
+    {
+        const int64_t *a = lwe_a + wid * N;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int k = t + i * T;
+            const int64_t ak = rhe_shift(tmod((uint64_t)a[k == 0 ? 0 : N - k], lq), LOGN);
+            A[i] = k == 0 ? (uint64_t)ak : 0ULL - (uint64_t)ak;
+            B[i] = 0;
+        }
+        if (t == 0)
+            B[0] = (uint64_t)rhe_shift(tmod((uint64_t)lwe_b[wid], lq), LOGN);
+    }
+    const double q = ldexp(1.0, lq), inv_q = ldexp(1.0, -lq);
+
+#pragma unroll 1
+    for (int st = 0; st < LOGN; st++) {
+        const int kexp = (1 << (LOGN - st)) + 1;
+        // ---- automorphism X -> X^k (polynomial_jax.py:171-200)
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int j = t + i * T;
+            const int e = (j * kexp) & (2 * N - 1);
+            const int tg = e & (N - 1);
+            const bool neg = e >= N;
+            xb[tg] = (int64_t)(neg ? 0ULL - A[i] : A[i]);
+            xb[N + tg] = (int64_t)(neg ? 0ULL - B[i] : B[i]);
+        }
+        __syncthreads();
+        int64_t r[8];
+        uint64_t Bp[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            r[i] = tmod((uint64_t)xb[t + i * T], lq);
+            Bp[i] = (uint64_t)xb[N + t + i * T];
+        }
+        // ---- gadget product with the Galois key of this step
+        const cplx *key = gkey + (size_t)st * l * 2 * M;
+        cplx acc[2][2][2];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++)
+                acc[a][pp][0] = acc[a][pp][1] = {0.0, 0.0};
+#pragma unroll 1
+        for (int lev = 0; lev < l; lev += 2) {
+            cplx v[2][4];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int lv = lev + c;
+                double dg[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    dg[i] = lv < l ? (double)next_digit(r[i], lq - (lv + 1) * lb) : 0.0;
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++)
+                    v[c][rr] = fold_point<N>(dg[rr], dg[rr + 4], t, rr, ztab);
+            }
+            forward_any<N>(v, buf, ztab, t);
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++) {
+                const int rb = t + pp * T;   // bin pair (rb, M-1-rb) = F-bins 2rb, 2rb+1
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (lev + c >= l)
+                        break;
+                    const cplx d0 = buf[c * M + phys<N>(rb)];
+                    const cplx d1 = buf[c * M + phys<N>(M - 1 - rb)];
+#pragma unroll
+                    for (int a = 0; a < 2; a++) {
+                        const cplx *kb = key + ((size_t)(lev + c) * 2 + a) * M;
+                        acc[a][pp][0] = cfma(d0, kb[2 * rb], acc[a][pp][0]);
+                        acc[a][pp][1] = cfma_conj(kb[2 * rb + 1], d1, acc[a][pp][1]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pp = 0; pp < 2; pp++) {
+            const int rb = t + pp * T;
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                buf[a * M + phys<N>(rb)] = acc[a][pp][0];
+                buf[a * M + phys<N>(M - 1 - rb)] = acc[a][pp][1];
+            }
+        }
+        __syncthreads();
+        cplx wv[2][4];
+        inverse_any<N>(wv, buf, ztab, t);
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            double lo, hi;
+            unfold_point<N>(wv[0][rr], t, rr, ztab, lo, hi);
+            A[rr] += (uint64_t)f64_to_torus(lo, q, inv_q, lq);
+            A[rr + 4] += (uint64_t)f64_to_torus(hi, q, inv_q, lq);
+            unfold_point<N>(wv[1][rr], t, rr, ztab, lo, hi);
+            B[rr] += Bp[rr] + (uint64_t)f64_to_torus(lo, q, inv_q, lq);
+            B[rr + 4] += Bp[rr + 4] + (uint64_t)f64_to_torus(hi, q, inv_q, lq);
+        }
+    }
+    int64_t *dst = out + wid * 2 * N;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        dst[t + i * T] = tmod(A[i], lq);
+        dst[N + t + i * T] = tmod(B[i], lq);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plaintext-LUT products (prepare_LUT, activation.py:103-106, summed over the
+// many-LUT slots as Spike_LSTM.py:129-132 does), for a batch of sequences
+// sharing the LUTs:
+//   out[s, g] = sum_k round(LUT[k, g](X) * RLWE[s, k, g])   (both halves), mod q
+// lut_f holds the LUT polynomials already transformed (reference bin order).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(N / 8) void lut_product_kernel(const int64_t *__restrict__ rlwe,
+                                                            const cplx *__restrict__ lut_f,
+                                                            int64_t G, int n_lut, int lq,
+                                                            int64_t *__restrict__ out)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T;
+    __shared__ cplx buf[2 * M];
+    __shared__ cplx ztab[N];
+    const int t = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    const int64_t sq = blockIdx.y;
+    fill_root_table<N>(ztab);
+    const double q = ldexp(1.0, lq), inv_q = ldexp(1.0, -lq);
+    uint64_t S[2][8];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            S[a][i] = 0;
+    __syncthreads();
+#pragma unroll 1
+    for (int k = 0; k < n_lut; k++) {
+        const int64_t idx = (int64_t)k * G + g;
+        const int64_t *src = rlwe + ((sq * n_lut + k) * G + g) * 2 * N;
+        cplx v[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                v[a][rr] = fold_point<N>((double)src[a * N + t + rr * T],
+                                         (double)src[a * N + t + (rr + 4) * T], t, rr, ztab);
+        forward_any<N>(v, buf, ztab, t);
+        const cplx *L = lut_f + idx * M;
+#pragma unroll
+        for (int pp = 0; pp < 2; pp++) {
+            const int rb = t + pp * T;
+            const cplx l0 = L[2 * rb], l1 = cconj(L[2 * rb + 1]);
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int p0 = a * M + phys<N>(rb), p1 = a * M + phys<N>(M - 1 - rb);
+                buf[p0] = cmul(buf[p0], l0);
+                buf[p1] = cmul(buf[p1], l1);
+            }
+        }
+        __syncthreads();
+        cplx wv[2][4];
+        inverse_any<N>(wv, buf, ztab, t);
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                double lo, hi;
+                unfold_point<N>(wv[a][rr], t, rr, ztab, lo, hi);
+                S[a][rr] += (uint64_t)f64_to_torus(lo, q, inv_q, lq);
+                S[a][rr + 4] += (uint64_t)f64_to_torus(hi, q, inv_q, lq);
+            }
+    }
+    int64_t *dst = out + (sq * G + g) * 2 * N;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            dst[a * N + t + i * T] = tmod(S[a][i], lq);
+}
+
+// ---------------------------------------------------------------------------
+// Negacyclic transform of integer polynomials, output in the reference's
+// bin order F_j = p(zeta^(2j+1)), j < N/2 (jax_fourier, polynomial_jax.py:11-15).
+// Two polynomials per workgroup.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(N / 8) void poly_fft_kernel(const int64_t *__restrict__ polys,
+                                                         int64_t cnt, cplx *__restrict__ out)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T;
+    __shared__ cplx buf[2 * M];
+    __shared__ cplx ztab[N];
+    const int t = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * 2;
+    fill_root_table<N>(ztab);
+    __syncthreads();
+    cplx v[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const bool ok = p0 + c < cnt;
+        const int64_t *src = polys + (p0 + c) * N;
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+            v[c][rr] = ok ? fold_point<N>((double)src[t + rr * T], (double)src[t + (rr + 4) * T],
+                                          t, rr, ztab)
+                          : cplx{0.0, 0.0};
+    }
+    forward_any<N>(v, buf, ztab, t);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        if (p0 + c >= cnt)
+            break;
+        cplx *dst = out + (p0 + c) * M;
+#pragma unroll
+        for (int pp = 0; pp < 2; pp++) {
+            const int rb = t + pp * T;
+            dst[2 * rb] = buf[c * M + phys<N>(rb)];
+            dst[2 * rb + 1] = cconj(buf[c * M + phys<N>(M - 1 - rb)]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Packing of n_lwes LWEs (masks of length n_in) into one RLWE
+// (packing, text_jax.py:211-239):
+//   row j (j < n_in) is the polynomial P_j with coefficient i = a_i[j]
+//   (i < n_lwes, zero above);
+//   A = -sum_j sum_lev dec_lev(P_j) * KSP[j, lev, 0]
+//   B = (b_0 .. b_{n_lwes-1}, 0 ..) - sum_j sum_lev dec_lev(P_j) * KSP[j, lev, 1]
+// 1. pack_transpose_kernel: masks -> rows P_j (coalesced tile transpose)
+// 2. packing_mac_kernel: each workgroup transforms the digits of a block of
+//    rows and accumulates digit x key per bin; one partial sum per block
+// 3. packing_finalize_kernel: sums the partials in block order, one inverse
+//    transform per half, rounding, body.
+// The sum over rows is rounded once, after the transform domain sum (the
+// reference rounds per row, then sums).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pack_transpose_kernel(const int64_t *__restrict__ masks,
+                                                             int n_lwes, int n_in, int N, int lq,
+                                                             int64_t *__restrict__ rows)
+{
+    __shared__ int64_t tile[64][65];
+    const int p = blockIdx.z;
+    const int i0 = blockIdx.y * 64;   // coefficient (LWE index)
+    const int j0 = blockIdx.x * 64;   // row (mask coordinate)
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int ii = e >> 6, jj = e & 63;
+        const int i = i0 + ii, j = j0 + jj;
+        tile[ii][jj] = (i < n_lwes && j < n_in)
+                           ? tmod((uint64_t)masks[((size_t)p * n_lwes + i) * n_in + j], lq)
+                           : 0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int jj = e >> 6, ii = e & 63;
+        const int i = i0 + ii, j = j0 + jj;
+        if (j < n_in && i < N)
+            rows[((size_t)p * n_in + j) * N + i] = tile[ii][jj];
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8) void packing_mac_kernel(const int64_t *__restrict__ rows,
+                                                            int n_in, int rows_per_block,
+                                                            const cplx *__restrict__ ksk, int l,
+                                                            int lb, int lq,
+                                                            cplx *__restrict__ partial)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T;
+    __shared__ cplx buf[2 * M];
+    __shared__ cplx ztab[N];
+    const int t = threadIdx.x;
+    const int p = blockIdx.y, blk = blockIdx.x, NB = gridDim.x;
+    fill_root_table<N>(ztab);
+    __syncthreads();
+    const int j0 = blk * rows_per_block;
+    const int j1 = min(n_in, j0 + rows_per_block);
+    cplx acc[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int pp = 0; pp < 2; pp++)
+            acc[a][pp][0] = acc[a][pp][1] = {0.0, 0.0};
+#pragma unroll 1
+    for (int j = j0; j < j1; j++) {
+        const int64_t *row = rows + ((size_t)p * n_in + j) * N;
+        int64_t r[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            r[i] = row[t + i * T];
+#pragma unroll 1
+        for (int lev = 0; lev < l; lev += 2) {
+            cplx v[2][4];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int lv = lev + c;
+                double dg[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    dg[i] = lv < l ? (double)next_digit(r[i], lq - (lv + 1) * lb) : 0.0;
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++)
+                    v[c][rr] = fold_point<N>(dg[rr], dg[rr + 4], t, rr, ztab);
+            }
+            forward_any<N>(v, buf, ztab, t);
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++) {
+                const int rb = t + pp * T;
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (lev + c >= l)
+                        break;
+                    const cplx d0 = buf[c * M + phys<N>(rb)];
+                    const cplx d1 = buf[c * M + phys<N>(M - 1 - rb)];
+#pragma unroll
+                    for (int a = 0; a < 2; a++) {
+                        const cplx *kb = ksk + (((size_t)j * l + lev + c) * 2 + a) * M;
+                        acc[a][pp][0] = cfma(d0, kb[2 * rb], acc[a][pp][0]);
+                        acc[a][pp][1] = cfma_conj(kb[2 * rb + 1], d1, acc[a][pp][1]);
+                    }
+                }
+            }
+        }
+    }
+    // partial (P, NB, 2, M), U-domain bins in natural order
+    cplx *dst = partial + ((size_t)p * NB + blk) * 2 * M;
+#pragma unroll
+    for (int pp = 0; pp < 2; pp++) {
+        const int rb = t + pp * T;
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            dst[a * M + rb] = acc[a][pp][0];
+            dst[a * M + M - 1 - rb] = acc[a][pp][1];
+        }
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8) void packing_finalize_kernel(const cplx *__restrict__ partial,
+                                                                 int NB,
+                                                                 const int64_t *__restrict__ bodies,
+                                                                 int n_lwes, int lq,
+                                                                 int64_t *__restrict__ out)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T;
+    __shared__ cplx buf[2 * M];
+    __shared__ cplx ztab[N];
+    const int t = threadIdx.x;
+    const int p = blockIdx.x;
+    fill_root_table<N>(ztab);
+    const cplx *src = partial + (size_t)p * NB * 2 * M;
+#pragma unroll
+    for (int pp = 0; pp < 2; pp++) {
+        const int rb = t + pp * T;
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            cplx s0 = {0.0, 0.0}, s1 = {0.0, 0.0};
+            for (int blk = 0; blk < NB; blk++) {
+                s0 = cadd(s0, src[((size_t)blk * 2 + a) * M + rb]);
+                s1 = cadd(s1, src[((size_t)blk * 2 + a) * M + M - 1 - rb]);
+            }
+            buf[a * M + phys<N>(rb)] = s0;
+            buf[a * M + phys<N>(M - 1 - rb)] = s1;
+        }
+    }
+    __syncthreads();
+    cplx wv[2][4];
+    inverse_any<N>(wv, buf, ztab, t);
+    const double q = ldexp(1.0, lq), inv_q = ldexp(1.0, -lq);
+    int64_t *dst = out + (size_t)p * 2 * N;
+    const int64_t *bp = bodies + (size_t)p * n_lwes;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+        double lo, hi;
+        unfold_point<N>(wv[0][rr], t, rr, ztab, lo, hi);
+        const int k0 = t + rr * Fold<N>::T, k1 = k0 + N / 2;
+        dst[k0] = tmod(0ULL - (uint64_t)f64_to_torus(lo, q, inv_q, lq), lq);
+        dst[k1] = tmod(0ULL - (uint64_t)f64_to_torus(hi, q, inv_q, lq), lq);
+        unfold_point<N>(wv[1][rr], t, rr, ztab, lo, hi);
+        const uint64_t b0 = k0 < n_lwes ? (uint64_t)bp[k0] : 0ULL;
+        const uint64_t b1 = k1 < n_lwes ? (uint64_t)bp[k1] : 0ULL;
+        dst[N + k0] = tmod(b0 - (uint64_t)f64_to_torus(lo, q, inv_q, lq), lq);
+        dst[N + k1] = tmod(b1 - (uint64_t)f64_to_torus(hi, q, inv_q, lq), lq);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ciphertext arithmetic: out = centred(sx*x + sy*y) (sum_ciphertext_ciphertext,
+// text_jax.py:27-43) and out = X^e * in per polynomial (coef_rotation,
+// polynomial_jax.py:150-169; rotate_ciphertext, text_jax.py:348-362).
+// ---------------------------------------------------------------------------
+__global__ void torus_axpy_kernel(const int64_t *__restrict__ x, const int64_t *__restrict__ y,
+                                  int64_t n, int64_t sx, int64_t sy, int lq,
+                                  int64_t *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = tmod((uint64_t)sx * (uint64_t)x[i] + (uint64_t)sy * (uint64_t)y[i], lq);
+}
+
+__global__ void rotate_poly_kernel(const int64_t *__restrict__ in, int64_t rows, int N,
+                                   const int64_t *__restrict__ exps, int64_t exp_every, int lq,
+                                   int64_t *__restrict__ out)
+{
+    const int64_t row = blockIdx.y;
+    const int64_t e = exps[row / exp_every];
+    const int twoN = 2 * N;
+    const int al = (int)(((e % twoN) + twoN) % twoN);
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < N; s += gridDim.x * blockDim.x) {
+        // (X^al * p)[s] = p[w] if w < N else -p[w - N], w = (s - al) mod 2N
+        int w = s - al;
+        if (w < 0)
+            w += twoN;
+        const uint64_t v = (uint64_t)(w < N ? in[row * N + w] : in[row * N + w - N]);
+        out[row * N + s] = tmod(w < N ? v : 0ULL - v, lq);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+#define BSLSTM_DEGREE_SWITCH(deg, CALL)                                                          \
+    switch (deg) {                                                                               \
+    case 128: { constexpr int NN = 128; CALL; } break;                                           \
+    case 256: { constexpr int NN = 256; CALL; } break;                                           \
+    case 512: { constexpr int NN = 512; CALL; } break;                                           \
+    case 1024: { constexpr int NN = 1024; CALL; } break;                                         \
+    case 2048: { constexpr int NN = 2048; CALL; } break;                                         \
+    case 4096: { constexpr int NN = 4096; CALL; } break;                                         \
+    default: return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");                         \
+    }
+
+static bool degree_ok(int64_t d) { return d >= 128 && d <= 4096 && is_pow2(d); }
+
+static int check_gadget(int64_t l, int64_t log2_beta, int64_t log2_q)
+{
+    if (log2_q < 2 || log2_q > 64)
+        return fail(BSLSTM_INVALID_ARGUMENT, "log2_q must be in [2, 64]");
+    if (l < 1 || log2_beta < 1 || log2_beta > 30 || l * log2_beta > log2_q)
+        return fail(BSLSTM_INVALID_ARGUMENT, "need l >= 1, 1 <= log2_beta <= 30, l*log2_beta <= log2_q");
+    return BSLSTM_OK;
+}
+
+}  // namespace bslstm
+
+using namespace bslstm;
+
+extern "C" int bslstm_linear_extract(void *stream, const int64_t *ct, int64_t ct_elems,
+                                     const int64_t *weights, const int32_t *support, int64_t K,
+                                     const int64_t *bias, int64_t batch, int64_t out_dim,
+                                     int64_t degree, int64_t log2_q, int64_t *masks, int64_t *bodies)
+{
+    clear_error();
+    if (!is_pow2(degree) || degree < 2 || degree > 4096)
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (log2_q < 2 || log2_q > 64)
+        return fail(BSLSTM_INVALID_ARGUMENT, "log2_q must be in [2, 64]");
+    if (batch < 0 || batch > 65535 || out_dim < 0 || K < 0 || K > degree)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad batch/out_dim/K");
+    int64_t stride;
+    if (ct_elems == 2 * degree)
+        stride = 0;
+    else if (ct_elems == batch * 2 * degree)
+        stride = 2 * degree;
+    else
+        return fail(BSLSTM_INVALID_ARGUMENT, "ct has %lld elements; expected 2N or batch*2N",
+                    (long long)ct_elems);
+    if (batch == 0 || out_dim == 0)
+        return BSLSTM_OK;
+    if (!ct || !weights || (K > 0 && !support) || !masks || !bodies)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const int N = (int)degree;
+    dim3 grid((unsigned)((N + LIN_THREADS - 1) / LIN_THREADS),
+              (unsigned)((out_dim + LIN_OT - 1) / LIN_OT), (unsigned)batch);
+    if (grid.y > 65535)
+        return fail(BSLSTM_INVALID_ARGUMENT, "out_dim too large");
+    hipLaunchKernelGGL(linear_extract_kernel, grid, dim3(LIN_THREADS), sizeof(int64_t) * (2 * N),
+                       (hipStream_t)stream, ct, stride, weights, support, (int)K, bias,
+                       (int)out_dim, N, (int)log2_q, masks, bodies);
+    return launch_status("linear extract");
+}
+
+extern "C" int bslstm_pack_lwe_to_rlwe(void *stream, const int64_t *lwe_a, const int64_t *lwe_b,
+                                       int64_t count, const double *galois_key_fft,
+                                       int64_t degree, int64_t log2_beta, int64_t l,
+                                       int64_t log2_q, int64_t *out)
+{
+    clear_error();
+    if (!degree_ok(degree))
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (int s = check_gadget(l, log2_beta, log2_q))
+        return s;
+    if (count < 0 || count > 0x7fffffffLL)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad count");
+    if (count == 0)
+        return BSLSTM_OK;
+    if (!lwe_a || !lwe_b || !galois_key_fft || !out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const cplx *gk = reinterpret_cast<const cplx *>(galois_key_fft);
+    BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((trace_pack_kernel<NN>), dim3((unsigned)count),
+                                                    dim3(NN / 8), 0, (hipStream_t)stream, lwe_a,
+                                                    lwe_b, gk, (int)l, (int)log2_beta,
+                                                    (int)log2_q, out));
+    return launch_status("trace packing");
+}
+
+extern "C" int bslstm_lut_product_sum(void *stream, const int64_t *rlwe, const double *lut_fft,
+                                      int64_t batch, int64_t groups, int64_t n_lut, int64_t degree,
+                                      int64_t log2_q, int64_t *out)
+{
+    clear_error();
+    if (!degree_ok(degree))
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (log2_q < 2 || log2_q > 64)
+        return fail(BSLSTM_INVALID_ARGUMENT, "log2_q must be in [2, 64]");
+    if (groups < 0 || groups > 0x7fffffffLL || n_lut < 1 || batch < 0 || batch > 65535)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad batch/groups/n_lut (batch <= 65535)");
+    if (groups == 0 || batch == 0)
+        return BSLSTM_OK;
+    if (!rlwe || !lut_fft || !out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const cplx *lf = reinterpret_cast<const cplx *>(lut_fft);
+    BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((lut_product_kernel<NN>),
+                                                    dim3((unsigned)groups, (unsigned)batch),
+                                                    dim3(NN / 8), 0, (hipStream_t)stream, rlwe, lf,
+                                                    groups, (int)n_lut, (int)log2_q, out));
+    return launch_status("LUT product");
+}
+
+extern "C" int bslstm_poly_fft(void *stream, const int64_t *polys, int64_t count, int64_t degree,
+                               double *out)
+{
+    clear_error();
+    if (!degree_ok(degree))
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (count < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad count");
+    if (count == 0)
+        return BSLSTM_OK;
+    if (!polys || !out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const int64_t blocks = (count + 1) / 2;
+    if (blocks > 0x7fffffffLL)
+        return fail(BSLSTM_INVALID_ARGUMENT, "count too large");
+    cplx *o = reinterpret_cast<cplx *>(out);
+    BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((poly_fft_kernel<NN>), dim3((unsigned)blocks),
+                                                    dim3(NN / 8), 0, (hipStream_t)stream, polys,
+                                                    count, o));
+    return launch_status("polynomial transform");
+}
+
+static int packing_blocks(int64_t n_in) { return (int)std::min<int64_t>(n_in, 128); }
+
+extern "C" int64_t bslstm_packing_workspace_bytes(int64_t packs, int64_t n_in, int64_t degree)
+{
+    if (packs < 0 || n_in < 1 || !is_pow2(degree))
+        return -1;
+    const int64_t nb = packing_blocks(n_in);
+    return packs * n_in * degree * 8 + packs * nb * 2 * (degree / 2) * 16;
+}
+
+extern "C" int bslstm_packing(void *stream, const int64_t *masks, const int64_t *bodies,
+                              int64_t packs, int64_t n_lwes, int64_t n_in, const double *ksk_fft,
+                              int64_t degree, int64_t log2_beta, int64_t l, int64_t log2_q,
+                              void *workspace, int64_t workspace_bytes, int64_t *out)
+{
+    clear_error();
+    if (!degree_ok(degree))
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (int s = check_gadget(l, log2_beta, log2_q))
+        return s;
+    if (packs < 0 || packs > 65535 || n_lwes < 0 || n_lwes > degree || n_in < 1 || n_in > (1 << 24))
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad packs/n_lwes/n_in (n_lwes <= degree)");
+    if (packs == 0)
+        return BSLSTM_OK;
+    const int64_t need = bslstm_packing_workspace_bytes(packs, n_in, degree);
+    if (workspace_bytes < need)
+        return fail(BSLSTM_INVALID_ARGUMENT, "workspace of %lld bytes; need %lld",
+                    (long long)workspace_bytes, (long long)need);
+    if ((n_lwes > 0 && (!masks || !bodies)) || !ksk_fft || !workspace || !out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const int N = (int)degree;
+    int64_t *rows = reinterpret_cast<int64_t *>(workspace);
+    cplx *partial = reinterpret_cast<cplx *>(rows + packs * n_in * degree);
+    const int nb = packing_blocks(n_in);
+    const int rpb = (int)((n_in + nb - 1) / nb);
+    hipStream_t s = (hipStream_t)stream;
+    dim3 tg((unsigned)((n_in + 63) / 64), (unsigned)(N / 64), (unsigned)packs);
+    hipLaunchKernelGGL(pack_transpose_kernel, tg, dim3(256), 0, s, masks, (int)n_lwes, (int)n_in, N,
+                       (int)log2_q, rows);
+    if (int st = launch_status("packing transpose"))
+        return st;
+    const cplx *kf = reinterpret_cast<const cplx *>(ksk_fft);
+    BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((packing_mac_kernel<NN>), dim3(nb, (unsigned)packs),
+                                                    dim3(NN / 8), 0, s, rows, (int)n_in, rpb, kf,
+                                                    (int)l, (int)log2_beta, (int)log2_q, partial));
+    if (int st = launch_status("packing MAC"))
+        return st;
+    BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((packing_finalize_kernel<NN>), dim3((unsigned)packs),
+                                                    dim3(NN / 8), 0, s, partial, nb, bodies,
+                                                    (int)n_lwes, (int)log2_q, out));
+    return launch_status("packing finalize");
+}
+
+extern "C" int bslstm_torus_axpy(void *stream, const int64_t *x, const int64_t *y, int64_t n,
+                                 int64_t sx, int64_t sy, int64_t log2_q, int64_t *out)
+{
+    clear_error();
+    if (log2_q < 2 || log2_q > 64 || n < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad log2_q/n");
+    if (n == 0)
+        return BSLSTM_OK;
+    if (!x || !y || !out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const int threads = 256;
+    const int64_t blocks = std::min<int64_t>((n + threads - 1) / threads, 1 << 16);
+    hipLaunchKernelGGL(torus_axpy_kernel, dim3((unsigned)blocks), dim3(threads), 0,
+                       (hipStream_t)stream, x, y, n, sx, sy, (int)log2_q, out);
+    return launch_status("torus axpy");
+}
+
+extern "C" int bslstm_rotate_poly(void *stream, const int64_t *in, int64_t rows, int64_t degree,
+                                  const int64_t *exponents, int64_t rows_per_exponent,
+                                  int64_t log2_q, int64_t *out)
+{
+    clear_error();
+    if (!is_pow2(degree) || degree < 2 || degree > (1 << 20))
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (log2_q < 2 || log2_q > 64 || rows < 0 || rows > 65535 || rows_per_exponent < 1)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad log2_q/rows (<= 65535)/rows_per_exponent");
+    if (rows == 0)
+        return BSLSTM_OK;
+    if (!in || !exponents || !out || in == out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL or aliased operand");
+    const int threads = 256;
+    const int bx = (int)std::min<int64_t>((degree + threads - 1) / threads, 64);
+    hipLaunchKernelGGL(rotate_poly_kernel, dim3(bx, (unsigned)rows), dim3(threads), 0,
+                       (hipStream_t)stream, in, rows, (int)degree, exponents, rows_per_exponent,
+                       (int)log2_q, out);
+    return launch_status("polynomial rotation");
+}

@@ -153,6 +153,155 @@ __global__ __launch_bounds__(KS_THREADS) void key_switch_kernel(
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// int64 forms for moduli above 2^53 (the reference's q = 2^64 LSTM
+// parameters, parameters_crypto.py:6): float64 cannot hold those classes, so
+// ciphertexts are int64 centred representatives and sums wrap mod 2^64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t tmod64(uint64_t x, int lq)
+{
+    const int s = 64 - lq;
+    return (int64_t)(x << s) >> s;
+}
+
+__device__ __forceinline__ int64_t rhe_shift64(int64_t r, int s)
+{
+    const int64_t fl = r >> s;
+    const uint64_t rem = (uint64_t)r & ((1ULL << s) - 1);
+    const uint64_t half = 1ULL << (s - 1);
+    return fl + (int64_t)((rem > half) | ((rem == half) & ((uint64_t)fl & 1)));
+}
+
+// rotation exponents from int64 masks: the group sum is formed exactly
+// (mod 2^64, then centred mod q), then the reference's FP64 rounding
+// round(2N*val/(q*n_lut))*n_lut mod 2N (multiply.py:67-79).
+__global__ void rotation_indices_i64_kernel(const int64_t *__restrict__ pk, int64_t total, int n,
+                                            int c, int m, int n_iter, int lq, double two_n,
+                                            double q_nlut, double n_lut, int two_N,
+                                            int32_t *__restrict__ out)
+{
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total)
+        return;
+    const int p = (int)(gid % m);
+    const int64_t bg = gid / m;
+    const int g = (int)(bg % n_iter);
+    const int64_t b = bg / n_iter;
+    const int64_t *row = pk + b * n + (int64_t)g * c;
+    uint64_t acc = 0;
+    for (int j = 0; j < c; j++)
+        if ((p >> (c - 1 - j)) & 1)
+            acc += (uint64_t)row[j];
+    const double val = (double)tmod64(acc, lq);
+    const int32_t v = (int32_t)(rint((two_n * val) / q_nlut) * n_lut);
+    int32_t r = v % two_N;
+    if (r < 0)
+        r += two_N;
+    out[gid] = r;
+}
+
+// X^(-b_ms) rotation (skipped when body is NULL) + extraction of coefficients
+// 0..n_ext-1 (RLWE_jax.py:295-303, text_jax.py:191-208), int64 in and out
+__global__ void rotate_extract_i64_kernel(const int64_t *__restrict__ br,
+                                          const int64_t *__restrict__ body, int N, int n_ext,
+                                          double two_n, double q_nlut, double nl, int lq,
+                                          int64_t *__restrict__ masks, int64_t *__restrict__ bodies)
+{
+    const int i = blockIdx.y;
+    const int k = blockIdx.z;
+    const int64_t *A = br + (size_t)i * 2 * N;
+    const int64_t *Bp = A + N;
+    const int alpha = body ? -(int)(rint((two_n * (double)body[i]) / q_nlut) * nl) : 0;
+    const int twoN = 2 * N;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+        int s = k - j;
+        bool neg = false;
+        if (s < 0) {
+            s += N;
+            neg = true;
+        }
+        int w = (s - alpha) % twoN;
+        if (w < 0)
+            w += twoN;
+        uint64_t v = (uint64_t)(w < N ? A[w] : A[w - N]);
+        if ((w >= N) != neg)
+            v = 0ULL - v;
+        masks[((size_t)i * n_ext + k) * N + j] = tmod64(v, lq);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int w = (k - alpha) % twoN;
+        if (w < 0)
+            w += twoN;
+        const uint64_t v = w < N ? (uint64_t)Bp[w] : 0ULL - (uint64_t)Bp[w - N];
+        bodies[(size_t)i * n_ext + k] = tmod64(v, lq);
+    }
+}
+
+// exact int64 key switch: a' = sum d*KSK_a, b' = b + sum d*KSK_b mod 2^log2_q,
+// digits by the float rule restated on integers (polynomial_jax.py:90-111)
+__global__ __launch_bounds__(KS_THREADS) void key_switch_i64_kernel(
+    const int64_t *__restrict__ a_in, const int64_t *__restrict__ b_in,
+    const int64_t *__restrict__ ksk_a, const int64_t *__restrict__ ksk_b, int batch, int n_in,
+    int n_out, int lb, int l_ks, int ic_max, int lq, int64_t *__restrict__ a_out,
+    int64_t *__restrict__ b_out)
+{
+    extern __shared__ int32_t digi[];   // [l_ks][KS_RB][ic_max]
+    const int j = blockIdx.x * KS_THREADS + threadIdx.x;
+    const int r0 = blockIdx.y * KS_RB;
+    const bool active = j <= n_out;
+    uint64_t acc[KS_RB];
+#pragma unroll
+    for (int r = 0; r < KS_RB; r++)
+        acc[r] = 0;
+    for (int i0 = 0; i0 < n_in; i0 += ic_max) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < KS_RB * ic_max; e += KS_THREADS) {
+            const int r = e / ic_max, ii = e % ic_max;
+            const int row = r0 + r, i = i0 + ii;
+            int64_t x = (row < batch && i < n_in) ? tmod64((uint64_t)a_in[(size_t)row * n_in + i], lq) : 0;
+            for (int lev = 0; lev < l_ks; lev++) {
+                const int sh = lq - (lev + 1) * lb;
+                int64_t d;
+                if (sh <= 0) {
+                    d = x;
+                    x = 0;
+                } else {
+                    d = rhe_shift64(x, sh);
+                    x = (int64_t)((uint64_t)x - ((uint64_t)d << sh));
+                }
+                digi[(lev * KS_RB + r) * ic_max + ii] = (int32_t)d;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const int ic = min(ic_max, n_in - i0);
+            for (int lev = 0; lev < l_ks; lev++) {
+                for (int ii = 0; ii < ic; ii++) {
+                    const size_t kidx = (size_t)lev * n_in + i0 + ii;
+                    const uint64_t kv = (uint64_t)(j < n_out ? ksk_a[kidx * n_out + j] : ksk_b[kidx]);
+                    const int32_t *dr = digi + (lev * KS_RB) * ic_max + ii;
+#pragma unroll
+                    for (int r = 0; r < KS_RB; r++)
+                        acc[r] += (uint64_t)(int64_t)dr[r * ic_max] * kv;
+                }
+            }
+        }
+    }
+    if (!active)
+        return;
+#pragma unroll
+    for (int r = 0; r < KS_RB; r++) {
+        const int row = r0 + r;
+        if (row >= batch)
+            break;
+        if (j < n_out)
+            a_out[(size_t)row * n_out + j] = tmod64(acc[r], lq);
+        else
+            b_out[row] = tmod64((uint64_t)b_in[row] + acc[r], lq);
+    }
+}
+
 }  // namespace bslstm
 
 using namespace bslstm;
@@ -236,5 +385,87 @@ extern "C" int bslstm_key_switch_lwe(void *stream, const double *a_in, const dou
     hipLaunchKernelGGL(key_switch_kernel, grid, dim3(KS_THREADS), lds, (hipStream_t)stream, a_in,
                        b_in, ksk_a, ksk_b, (int)batch, (int)n_in, (int)n_out,
                        ldexp(1.0, (int)log2_beta_ks), (int)l_ks, ic_max, q, 1.0 / q, a_out, b_out);
+    return launch_status("key switch");
+}
+
+extern "C" int bslstm_rotation_indices_i64(void *stream, const int64_t *public_key, int64_t batch,
+                                           int64_t n, int64_t collapse, int64_t log2_q,
+                                           int64_t degree, int64_t n_lut, int32_t *indices)
+{
+    clear_error();
+    if (collapse < 1 || collapse > 6 || n % collapse != 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "n=%lld must be a multiple of collapse=%lld in [1,6]",
+                    (long long)n, (long long)collapse);
+    if (!is_pow2(degree) || degree > (1 << 20))
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (n_lut < 1 || log2_q < 2 || log2_q > 64 || batch < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad n_lut/log2_q/batch");
+    const int n_iter = (int)(n / collapse), m = 1 << collapse;
+    const int64_t total = batch * n_iter * m;
+    if (total == 0)
+        return BSLSTM_OK;
+    if (!public_key || !indices)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const double q = ldexp(1.0, (int)log2_q);
+    const int threads = 256;
+    const int64_t blocks = (total + threads - 1) / threads;
+    hipLaunchKernelGGL(rotation_indices_i64_kernel, dim3((unsigned)blocks), dim3(threads), 0,
+                       (hipStream_t)stream, public_key, total, (int)n, (int)collapse, m, n_iter,
+                       (int)log2_q, (double)(2 * degree), q * (double)n_lut, (double)n_lut,
+                       (int)(2 * degree), indices);
+    return launch_status("rotation indices");
+}
+
+extern "C" int bslstm_rotate_extract_i64(void *stream, const int64_t *br_out, const int64_t *body,
+                                         int64_t batch, int64_t degree, int64_t log2_q,
+                                         int64_t n_lut, int64_t n_ext, int64_t *masks,
+                                         int64_t *bodies)
+{
+    clear_error();
+    if (!is_pow2(degree) || degree > 65536)
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (n_lut < 1 || n_ext < 1 || n_ext > degree || n_ext > 65535 || log2_q < 2 || log2_q > 64 ||
+        batch < 0 || batch > 65535)
+        return fail(BSLSTM_INVALID_ARGUMENT,
+                    "bad n_lut/n_ext/log2_q/batch (n_ext <= degree, batch <= 65535 per call)");
+    if (batch == 0)
+        return BSLSTM_OK;
+    if (!br_out || !masks || !bodies)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const double q = ldexp(1.0, (int)log2_q);
+    const int threads = 256;
+    const int bx = (int)((degree + threads - 1) / threads);
+    hipLaunchKernelGGL(rotate_extract_i64_kernel, dim3(bx, (unsigned)batch, (unsigned)n_ext),
+                       dim3(threads), 0, (hipStream_t)stream, br_out, body, (int)degree,
+                       (int)n_ext, (double)(2 * degree), q * (double)n_lut, (double)n_lut,
+                       (int)log2_q, masks, bodies);
+    return launch_status("rotate/extract");
+}
+
+extern "C" int bslstm_key_switch_lwe_i64(void *stream, const int64_t *a_in, const int64_t *b_in,
+                                         const int64_t *ksk_a, const int64_t *ksk_b, int64_t batch,
+                                         int64_t n_in, int64_t n_out, int64_t log2_beta_ks,
+                                         int64_t l_ks, int64_t log2_q, int64_t *a_out,
+                                         int64_t *b_out)
+{
+    clear_error();
+    if (log2_q < 2 || log2_q > 64)
+        return fail(BSLSTM_INVALID_ARGUMENT, "log2_q must be in [2, 64]");
+    if (l_ks < 1 || l_ks > KS_LMAX || log2_beta_ks < 1 || log2_beta_ks > 30 ||
+        l_ks * log2_beta_ks > log2_q)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad l_ks/log2_beta_ks");
+    if (n_in < 1 || n_out < 1 || batch < 0 || batch > (1LL << 30))
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad shapes");
+    if (batch == 0)
+        return BSLSTM_OK;
+    if (!a_in || !b_in || !ksk_a || !ksk_b || !a_out || !b_out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    dim3 grid((unsigned)((n_out + 1 + KS_THREADS - 1) / KS_THREADS),
+              (unsigned)((batch + KS_RB - 1) / KS_RB));
+    const int ic_max = (int)std::min<int64_t>(32, (2 * KS_LDS_DOUBLES) / (l_ks * KS_RB));
+    const size_t lds = sizeof(int32_t) * (size_t)l_ks * KS_RB * ic_max;
+    hipLaunchKernelGGL(key_switch_i64_kernel, grid, dim3(KS_THREADS), lds, (hipStream_t)stream,
+                       a_in, b_in, ksk_a, ksk_b, (int)batch, (int)n_in, (int)n_out,
+                       (int)log2_beta_ks, (int)l_ks, ic_max, (int)log2_q, a_out, b_out);
     return launch_status("key switch");
 }

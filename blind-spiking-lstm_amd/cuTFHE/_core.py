@@ -35,6 +35,23 @@ SIGNATURES = {
     "bslstm_rotate_extract": ([_P, _P, _P, _I64, _I64, _I64, _I64, _P, _P], ctypes.c_int),
     "bslstm_key_switch_lwe": ([_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P],
                               ctypes.c_int),
+    "bslstm_rotation_indices_i64": ([_P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P],
+                                    ctypes.c_int),
+    "bslstm_rotate_extract_i64": ([_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _P],
+                                  ctypes.c_int),
+    "bslstm_key_switch_lwe_i64": ([_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P,
+                                   _P], ctypes.c_int),
+    "bslstm_linear_extract": ([_P, _P, _I64, _P, _P, _I64, _P, _I64, _I64, _I64, _I64, _P, _P],
+                              ctypes.c_int),
+    "bslstm_pack_lwe_to_rlwe": ([_P, _P, _P, _I64, _P, _I64, _I64, _I64, _I64, _P],
+                                ctypes.c_int),
+    "bslstm_lut_product_sum": ([_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P], ctypes.c_int),
+    "bslstm_poly_fft": ([_P, _P, _I64, _I64, _P], ctypes.c_int),
+    "bslstm_packing_workspace_bytes": ([_I64, _I64, _I64], ctypes.c_int64),
+    "bslstm_packing": ([_P, _P, _P, _I64, _I64, _I64, _P, _I64, _I64, _I64, _I64, _P, _I64, _P],
+                       ctypes.c_int),
+    "bslstm_torus_axpy": ([_P, _P, _P, _I64, _I64, _I64, _I64, _P], ctypes.c_int),
+    "bslstm_rotate_poly": ([_P, _P, _I64, _I64, _P, _I64, _I64, _P], ctypes.c_int),
 }
 
 

@@ -12,7 +12,11 @@ import numpy as np
 import torch
 
 from cuTFHE import _core
-from schemas.polynomial_jax import GLEV_polynomial, centered_mod
+from schemas.polynomial_jax import (GLEV_polynomial, centered_mod, float_to_torus, glev_messages,
+                                    torus_mod)
+
+# above this modulus float64 cannot hold torus classes: int64 carriers
+EXACT_FLOAT_LOG_Q = 53
 
 
 def rng_of(key) -> np.random.Generator:
@@ -40,8 +44,17 @@ def encrypt_LWE(m, sk, dict_params: dict, key):
     q = dict_params["q"]
     log_q = log2_exact(q)
     sigma = dict_params["sigma"]
-    m = np.asarray(m, dtype=np.float64)
     sk = np.asarray(sk, dtype=np.int64)
+    if log_q > EXACT_FLOAT_LOG_Q:
+        # int64 classes (see EXACT_FLOAT_LOG_Q): m as int64 or integer-valued float
+        m = np.asarray(m)
+        mi = m.astype(np.int64) if m.dtype.kind in "iu" else float_to_torus(m, log_q)
+        a = uniform_torus(rng, mi.shape + (len(sk),), log_q)
+        e = (np.rint(rng.standard_normal(mi.shape)) * sigma).astype(np.int64)
+        with np.errstate(over="ignore"):
+            b = torus_mod((a * sk).sum(axis=-1) + mi + e, log_q)
+        return a, b[..., None]
+    m = np.asarray(m, dtype=np.float64)
     a = uniform_torus(rng, m.shape + (len(sk),), log_q)
     e = np.rint(rng.standard_normal(m.shape)) * sigma
     dot = (a @ sk).astype(np.float64)          # exact int64 sum (wraps mod 2^64)
@@ -53,18 +66,30 @@ def decrypt_LWE(ciphertext, sk, dict_params: dict):
     """LWE_jax.py:30-48: round((b - <a,s>)/delta) centred mod t."""
     t = dict_params["t"]
     delta = dict_params["q"] / t
-    a, b = np.asarray(ciphertext[0], np.float64), np.asarray(ciphertext[1], np.float64)
-    phase = b.reshape(a.shape[:-1]) - a @ np.asarray(sk, np.float64)
-    return centered_mod(np.rint(phase / delta), t)
+    return centered_mod(np.rint(lwe_phase(ciphertext, sk, dict_params) / delta), t)
+
+
+def lwe_phase(ciphertext, sk, dict_params: dict) -> np.ndarray:
+    """Centred phase b - <a, s> mod q (float64 values; exact int64 arithmetic
+    for int64 ciphertexts)."""
+    a, b = ciphertext[0], ciphertext[1]
+    if isinstance(a, torch.Tensor):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+    a, b = np.asarray(a), np.asarray(b)
+    log_q = log2_exact(dict_params["q"])
+    if a.dtype == np.int64:
+        with np.errstate(over="ignore"):
+            ph = b.reshape(a.shape[:-1]).astype(np.int64) - (a * np.asarray(sk, np.int64)).sum(-1)
+        return torus_mod(ph, log_q).astype(np.float64)
+    phase = b.reshape(a.shape[:-1]).astype(np.float64) - a.astype(np.float64) @ np.asarray(sk, np.float64)
+    return centered_mod(phase, dict_params["q"])
 
 
 def decrypt_LWE_quantization(ciphertext, sk, dict_params: dict, scale: float | None = None):
     """LWE_jax.py:52-62: centred phase divided by scale."""
     q = dict_params["q"]
     scale = q / dict_params["t"] if scale is None else scale
-    a, b = np.asarray(ciphertext[0], np.float64), np.asarray(ciphertext[1], np.float64)
-    phase = b.reshape(a.shape[:-1]) - a @ np.asarray(sk, np.float64)
-    return centered_mod(phase, q) / scale
+    return lwe_phase(ciphertext, sk, dict_params) / scale
 
 
 def get_keyswitch_sk_LWE(new_sk, old_sk, dict_params: dict, key):
@@ -74,6 +99,9 @@ def get_keyswitch_sk_LWE(new_sk, old_sk, dict_params: dict, key):
     beta = dict_params["beta_ks"]
     l = dict_params["l_ks"]
     q = dict_params["q"]
+    if log2_exact(q) > EXACT_FLOAT_LOG_Q:
+        glev = glev_messages(-np.asarray(old_sk, np.int64), log2_exact(beta), l, log2_exact(q))
+        return encrypt_LWE(glev, new_sk, dict_params, key)
     glev = GLEV_polynomial(np.asarray(old_sk, np.float64), beta, l, q)    # (l, N_in)
     return encrypt_LWE(-glev, new_sk, dict_params, key)
 
@@ -92,6 +120,13 @@ def key_switch_LWE(key_switching_key, ciphertext, dict_params: dict):
     _core.require_device(ksk_a, ksk_b, a, b)
     n_in = a.shape[-1]
     n_out = ksk_a.shape[-1]
+    if ksk_a.dtype == torch.int64 or a.dtype == torch.int64 or log2_exact(q) > 32:
+        # exact int64 form (any log2 q <= 64)
+        from cuTFHE import torus
+        lead = a.shape[:-1]
+        a_out, b_out = torus.key_switch(ksk_a, ksk_b, a.reshape(-1, n_in), b.reshape(-1),
+                                        log2_exact(beta), l, log2_exact(q))
+        return a_out.reshape(*lead, n_out), b_out.reshape(*lead, 1)
     if tuple(ksk_a.shape) != (l, n_in, n_out):
         raise _core.TFHEInvalidArgument(f"KSK shape {tuple(ksk_a.shape)} != {(l, n_in, n_out)}")
     lead = a.shape[:-1]

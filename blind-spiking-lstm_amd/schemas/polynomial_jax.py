@@ -80,3 +80,67 @@ def coef_rotation(polynomial: torch.Tensor, alpha: torch.Tensor) -> torch.Tensor
     src = torch.where(w < N, w, w - N)
     sign = torch.where(w < N, 1, -1).to(polynomial.dtype)
     return torch.gather(polynomial.reshape(-1, N), 1, src.expand(polynomial.reshape(-1, N).shape[0], N)) * sign
+
+
+# ---------------------------------------------------------------------------
+# integer torus helpers (moduli up to 2^64, where float64 cannot hold classes)
+# ---------------------------------------------------------------------------
+
+
+def torus_mod(x, log_q: int) -> np.ndarray:
+    """Centred representative mod 2^log_q of int64 classes (integer form of
+    centered_mod, polynomial_jax.py:7-9); sums wrap mod 2^64 beforehand."""
+    x = np.asarray(x).astype(np.int64)
+    s = 64 - log_q
+    return ((x.astype(np.uint64) << np.uint64(s)).astype(np.int64) >> np.int64(s)).astype(np.int64)
+
+
+def float_to_torus(v, log_q: int) -> np.ndarray:
+    """int64 class mod 2^log_q of integer-valued float64 values, including
+    magnitudes at or above 2^63 (e.g. s*q/beta with q = 2^64)."""
+    v = np.asarray(v, np.float64)
+    r = v - 2.0 ** 64 * np.rint(v / 2.0 ** 64)            # [-2^63, 2^63], exact
+    r = np.where(r >= 2.0 ** 63, r - 2.0 ** 64, r)
+    return torus_mod(r.astype(np.int64), log_q)
+
+
+def glev_messages(x, log_beta: int, l: int, log_q: int) -> np.ndarray:
+    """GLEV_polynomial (polynomial_jax.py:115-134) in exact integers:
+    x * q/beta^i for i = 1..l, as int64 classes (l, *x.shape)."""
+    x = np.asarray(x).astype(np.int64)
+    if l * log_beta > log_q:
+        raise ValueError("l*log2(beta) must not exceed log2(q)")
+    return np.stack([torus_mod((x.astype(np.uint64) << np.uint64(log_q - i * log_beta)).astype(
+        np.int64), log_q) for i in range(1, l + 1)])
+
+
+def decomposition(gamma, beta, l: int, q) -> np.ndarray:
+    """Float digit recursion of polynomial_jax.py:90-111 (host form):
+    x = gamma/q; digit = round_half_even(x*beta); x = x*beta - digit."""
+    x = np.asarray(gamma, np.float64) / q
+    out = []
+    for _ in range(l):
+        xb = x * beta
+        d = np.rint(xb)
+        x = xb - d
+        out.append(d)
+    return np.stack(out)
+
+
+def apply_automorphism(polynomial, alpha: int) -> np.ndarray:
+    """p(X) -> p(X^alpha) in Z[X]/(X^N+1) (polynomial_jax.py:171-200)."""
+    p = np.asarray(polynomial)
+    N = p.shape[-1]
+    e = (np.arange(N) * int(alpha)) % (2 * N)
+    out = np.zeros_like(p)
+    with np.errstate(over="ignore"):
+        out[..., e % N] = np.where(e >= N, -p, p)
+    return out
+
+
+def weights_to_polynomial(weights) -> np.ndarray:
+    """polynomial_jax.py:137-148: -flip(w), last entry replaced by w[0]."""
+    w = np.asarray(weights).reshape(-1)
+    p = -w[::-1].copy()
+    p[-1] = w[0]
+    return p

@@ -131,6 +131,127 @@ int bslstm_key_switch_lwe(void *stream, const double *a_in, const double *b_in,
                           int64_t log2_beta_ks, int64_t l_ks, int64_t log2_q,
                           double *a_out, double *b_out);
 
+/* ------------------------------------------------------------------------
+ * int64 forms.  The reference runs its LSTM at q = 2^64 (parameters_crypto.py:6)
+ * with float64 ciphertexts, which cannot hold 64-bit classes; these entry
+ * points take int64 centred representatives (any class representative is
+ * accepted on input) for any log2_q <= 64, with integer sums wrapping mod 2^64.
+ * ------------------------------------------------------------------------ */
+
+/* bslstm_rotation_indices with int64 masks: the group sums are exact, then
+ * the same FP64 rounding (cuTFHE/multiply.py:67-79). */
+int bslstm_rotation_indices_i64(void *stream, const int64_t *public_key, int64_t batch, int64_t n,
+                                int64_t collapse, int64_t log2_q, int64_t degree, int64_t n_lut,
+                                int32_t *indices);
+
+/* bslstm_rotate_extract with int64 operands and a separate extraction count:
+ * rotate by X^(-b_ms), b_ms = rint(2N*b/(q*n_lut))*n_lut (no rotation when
+ * body is NULL), then sample-extract coefficients 0..n_ext-1.
+ * Replaces schemas/RLWE_jax.py:295-303 (n_ext = n_lut) and the hidden-state
+ * extraction vmap(sample_extract)(C, arange(hidden)) of
+ * nn_tfhe/Spike_LSTM.py:113-114 (body NULL, n_ext = hidden).
+ *   br_out int64 (batch, 2, N); body int64 (batch) or NULL;
+ *   masks int64 (batch, n_ext, N); bodies int64 (batch, n_ext). */
+int bslstm_rotate_extract_i64(void *stream, const int64_t *br_out, const int64_t *body,
+                              int64_t batch, int64_t degree, int64_t log2_q, int64_t n_lut,
+                              int64_t n_ext, int64_t *masks, int64_t *bodies);
+
+/* Exact LWE key switch for log2_q <= 64 (schemas/LWE_jax.py:97-117, 65-82;
+ * digits by the float rule of schemas/polynomial_jax.py:90-111 restated on
+ * integers).  Layouts as bslstm_key_switch_lwe, all int64. */
+int bslstm_key_switch_lwe_i64(void *stream, const int64_t *a_in, const int64_t *b_in,
+                              const int64_t *ksk_a, const int64_t *ksk_b,
+                              int64_t batch, int64_t n_in, int64_t n_out,
+                              int64_t log2_beta_ks, int64_t l_ks, int64_t log2_q,
+                              int64_t *a_out, int64_t *b_out);
+
+/*
+ * Homomorphic Linear layer with coefficient-0 extraction.
+ * Replaces Linear.__call__ (nn_tfhe/Spike_LSTM.py:46-61): for every output
+ * row o, multiply_plaintext_ciphertext(W_o, ct) (schemas/text_jax.py:59-78),
+ * sample_extract(., 0) (text_jax.py:191-208), + bias_o, centred mod q.
+ * The product is exact (the reference rounds an FP64 FFT product).
+ *   ct      int64 (batch, 2, N) or (2, N) broadcast (ct_elems = its size)
+ *   weights int64 (out_dim, N) weight polynomials (Linear.set_weights,
+ *           Spike_LSTM.py:27-41); support int32 (K) the columns that may be
+ *           nonzero in any row (all others are skipped); bias int64 (out_dim)
+ *           or NULL
+ *   masks   int64 (batch, out_dim, N); bodies int64 (batch, out_dim)
+ */
+int bslstm_linear_extract(void *stream, const int64_t *ct, int64_t ct_elems,
+                          const int64_t *weights, const int32_t *support, int64_t K,
+                          const int64_t *bias, int64_t batch, int64_t out_dim,
+                          int64_t degree, int64_t log2_q, int64_t *masks, int64_t *bodies);
+
+/*
+ * LWE -> RLWE trace packing, one RLWE per LWE.
+ * Replaces pack_lwe_to_rlwe (schemas/text_jax.py:314-345) with
+ * modulus_switch (:159-179), trivial_embedding (:268-287) and
+ * apply_automorphism_and_keyswitch (:290-311).
+ *   lwe_a int64 (count, N), lwe_b int64 (count)
+ *   galois_key_fft complex128 (log2 N, l, 2, N/2): get_galois_key
+ *           (schemas/RLWE_jax.py:217-226), reference bin order
+ *   out int64 (count, 2, N)
+ * Digits use log2_beta / l (dict_params_packing beta_ks / l_ks).
+ */
+int bslstm_pack_lwe_to_rlwe(void *stream, const int64_t *lwe_a, const int64_t *lwe_b,
+                            int64_t count, const double *galois_key_fft, int64_t degree,
+                            int64_t log2_beta, int64_t l, int64_t log2_q, int64_t *out);
+
+/*
+ * Plaintext-LUT products summed over the many-LUT slots, for a batch of
+ * sequences sharing the LUTs.
+ * Replaces vmap(vmap(prepare_LUT))'s product (nn_tfhe/activation.py:103-106,
+ * multiply_plaintext_ciphertext with fft_product_twice) and the slot sum of
+ * nn_tfhe/Spike_LSTM.py:129-132:
+ *   out[s, g] = centred(sum_k round(LUT[k, g] * RLWE[s, k, g]))
+ *   rlwe int64 (batch, n_lut, G, 2, N); lut_fft complex128 (n_lut, G, N/2)
+ *   (the LUT polynomials through bslstm_poly_fft); out int64 (batch, G, 2, N).
+ */
+int bslstm_lut_product_sum(void *stream, const int64_t *rlwe, const double *lut_fft,
+                           int64_t batch, int64_t groups, int64_t n_lut, int64_t degree,
+                           int64_t log2_q, int64_t *out);
+
+/* Negacyclic transform of integer polynomials into complex128 (count, N/2),
+ * bins F_j = p(zeta^(2j+1)), zeta = exp(-i pi/N): jax_fourier
+ * (schemas/polynomial_jax.py:11-15) for keys and LUTs. */
+int bslstm_poly_fft(void *stream, const int64_t *polys, int64_t count, int64_t degree,
+                    double *out);
+
+/* Workspace of bslstm_packing in bytes (-1 on bad arguments). */
+int64_t bslstm_packing_workspace_bytes(int64_t packs, int64_t n_in, int64_t degree);
+
+/*
+ * Packing of up to N LWEs into one RLWE with the packing key switching key.
+ * Replaces packing (schemas/text_jax.py:211-239); `packs` independent
+ * packings per call.
+ *   masks int64 (packs, n_lwes, n_in), bodies int64 (packs, n_lwes),
+ *   n_lwes <= N; LWE i lands on coefficient i.
+ *   ksk_fft complex128 (n_in, l, 2, N/2): get_packing_KSK
+ *           (schemas/RLWE_jax.py:205-214) through bslstm_poly_fft
+ *   workspace device memory of bslstm_packing_workspace_bytes() bytes
+ *   out int64 (packs, 2, N)
+ * The transform-domain sum over the n_in rows is rounded once (the
+ * reference rounds each row's product before summing).
+ */
+int bslstm_packing(void *stream, const int64_t *masks, const int64_t *bodies, int64_t packs,
+                   int64_t n_lwes, int64_t n_in, const double *ksk_fft, int64_t degree,
+                   int64_t log2_beta, int64_t l, int64_t log2_q, void *workspace,
+                   int64_t workspace_bytes, int64_t *out);
+
+/* out = centred(sx*x + sy*y mod q) elementwise over n int64 values:
+ * sum_ciphertext_ciphertext (schemas/text_jax.py:27-43) and its differences. */
+int bslstm_torus_axpy(void *stream, const int64_t *x, const int64_t *y, int64_t n, int64_t sx,
+                      int64_t sy, int64_t log2_q, int64_t *out);
+
+/* out[r] = X^e * in[r] in Z_q[X]/(X^N+1) for rows r < rows (int64 (rows, N)),
+ * e = exponents[r / rows_per_exponent]: coef_rotation
+ * (schemas/polynomial_jax.py:150-169) / rotate_ciphertext
+ * (schemas/text_jax.py:348-362).  in and out must not alias. */
+int bslstm_rotate_poly(void *stream, const int64_t *in, int64_t rows, int64_t degree,
+                       const int64_t *exponents, int64_t rows_per_exponent, int64_t log2_q,
+                       int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

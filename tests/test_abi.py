@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _core.lib().bslstm_abi_version() == 1
+    assert _core.lib().bslstm_abi_version() == 2
 
 
 def _mul(degree=2048, l=3, dims=(250, 2, 3, 2, 8, 1024), ct_elems=None, idx_elems=None, batch=4,

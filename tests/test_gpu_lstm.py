@@ -1,0 +1,324 @@
+"""Parity of the LSTM-path HIP kernels (through the C ABI) with the CPU oracle
+(oracle/lstm_oracle.py), and one encrypted spiking-LSTM timestep end to end.
+
+Bars:
+  * bit-exact: Linear + extraction, int64 key switch, int64 rotation
+    indices, int64 rotate/extract, torus sums, X^e rotations (exact integer
+    kernels);
+  * FP64-transform kernels against the exact oracle: LUT products and
+    packing (one decomposition of exact inputs) coefficient by coefficient
+    within TOL_REL * q (the FP64 rounding of 64-bit torus values; a wrong
+    index, sign or twiddle gives errors of order q), exactly equal at q = 2^32
+    sizes where the transform is exact;
+  * kernels that decompose their own FP64 results again (trace packing, the
+    blind rotation) at q = 2^64: there the FP64 error (~2^30) exceeds half
+    the last digit's step, so the next decomposition may pick the other digit
+    at a rounding boundary and add a key row encrypting (almost) zero.  The
+    ciphertext is then a different but equally valid encryption: the bar is
+    the PHASE (b - <a, s>), within TOL_REL * q of the oracle's.  At q = 2^32
+    the coefficients match exactly;
+  * the timestep (reference structure and moduli, small ring): decrypted
+    hidden and cell states equal the plaintext spiking-LSTM rule for every
+    unit whose gate pre-activations lie clear of their thresholds.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import exact as X
+from oracle import lstm_oracle as L
+from oracle import tfhe_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL = 2.0 ** -20      # FP64 transform products of 64-bit torus values
+
+
+def dev(x, dtype=None):
+    t = torch.as_tensor(np.ascontiguousarray(x))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to("cuda:0")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def max_err(a, b, log_q):
+    with np.errstate(over="ignore"):
+        d = L.tmod(np.asarray(a, np.int64) - np.asarray(b, np.int64), log_q)
+    return float(np.abs(d.astype(np.float64)).max(initial=0.0))
+
+
+@pytest.fixture(scope="module")
+def T():
+    from cuTFHE import torus
+    return torus
+
+
+def test_poly_fft_matches_reference_transform(cuda_device, T):
+    rng = np.random.default_rng(1)
+    for N in (128, 256, 512, 1024, 2048, 4096):
+        p = rng.integers(-2 ** 40, 2 ** 40, (3, N))
+        got = host(T.poly_fft(dev(p)))
+        want = O.negacyclic_fft(p.astype(np.float64))
+        assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max(), N
+
+
+@pytest.mark.parametrize("N,log_q,K,out,batch", [(256, 64, 40, 10, 3), (2048, 64, 384, 32, 2),
+                                                 (2048, 32, 256, 17, 1), (512, 48, 512, 9, 2)])
+def test_linear_extract_bit_exact(cuda_device, T, N, log_q, K, out, batch):
+    rng = np.random.default_rng(N + K)
+    W = rng.integers(-40, 41, (K, out))
+    P = L.weights_to_polynomial(W, N)
+    bias = rng.integers(-2 ** 20, 2 ** 20, out)
+    ct = L.uniform(rng, (batch, 2, N), log_q)
+    m, b = T.linear_extract(dev(ct), dev(P), dev(bias), log_q)
+    m, b = host(m), host(b)
+    for i in range(batch):
+        wm, wb = L.linear_extract(ct[i], P, bias, log_q)
+        np.testing.assert_array_equal(m[i], wm)
+        np.testing.assert_array_equal(b[i], wb)
+
+
+def test_linear_extract_broadcast_and_dense_support(cuda_device, T):
+    rng = np.random.default_rng(7)
+    N, log_q = 256, 64
+    P = rng.integers(-5, 6, (4, N))                    # dense weight polynomials
+    ct = L.uniform(rng, (2, N), log_q)
+    m, b = T.linear_extract(dev(ct), dev(P), None, log_q)
+    wm, wb = L.linear_extract(ct, P, None, log_q)
+    np.testing.assert_array_equal(host(m), wm)
+    np.testing.assert_array_equal(host(b), wb)
+
+
+@pytest.mark.parametrize("log_q,lb,l,n_in,n_out,B", [(64, 1, 20, 2048, 702, 40), (64, 4, 8, 300, 97, 70),
+                                                     (32, 4, 5, 1024, 630, 33)])
+def test_key_switch_i64_bit_exact(cuda_device, T, log_q, lb, l, n_in, n_out, B):
+    rng = np.random.default_rng(l * n_in)
+    s_old, s_new = rng.integers(0, 2, n_in), rng.integers(0, 2, n_out)
+    ka, kb = L.keyswitch_key(rng, s_new, s_old, log_q, lb, l, 2.0 ** (log_q - 20))
+    a, b = L.lwe_encrypt(rng, rng.integers(-8, 8, B) << (log_q - 5), s_old, log_q, 2.0 ** (log_q - 30))
+    ga, gb = T.key_switch(dev(ka), dev(kb), dev(a), dev(b), lb, l, log_q)
+    wa, wb = L.key_switch_int(ka, kb, a, b, log_q, lb, l)
+    np.testing.assert_array_equal(host(ga), wa)
+    np.testing.assert_array_equal(host(gb), wb)
+
+
+def test_rotation_indices_i64_bit_exact(cuda_device, T):
+    rng = np.random.default_rng(3)
+    for (N, n, c, n_lut, lq) in [(2048, 702, 3, 4, 64), (256, 48, 3, 4, 64), (1024, 630, 3, 1, 32)]:
+        pk = L.uniform(rng, (50, n), lq)
+        got = host(T.rotation_indices(dev(pk), lq, N, c, n_lut))
+        np.testing.assert_array_equal(got, L.rotation_indices_int(pk, lq, N, c, n_lut))
+
+
+def test_rotate_extract_i64_bit_exact(cuda_device, T):
+    rng = np.random.default_rng(4)
+    for (N, lq, n_lut, n_ext, with_body) in [(2048, 64, 4, 4, True), (256, 64, 4, 4, True),
+                                             (2048, 64, 1, 128, False), (512, 32, 2, 2, True)]:
+        br = L.uniform(rng, (9, 2, N), lq)
+        body = L.uniform(rng, 9, lq) if with_body else None
+        gm, gb = T.rotate_extract(dev(br), None if body is None else dev(body), lq, n_lut, n_ext)
+        wm, wb = L.rotate_extract_int(br, body, lq, n_lut, n_ext)
+        np.testing.assert_array_equal(host(gm), wm)
+        np.testing.assert_array_equal(host(gb), wb)
+
+
+def test_axpy_and_rotate_bit_exact(cuda_device, T):
+    rng = np.random.default_rng(5)
+    for lq in (64, 40):
+        x, y = L.uniform(rng, (4, 2, 512), lq), L.uniform(rng, (4, 2, 512), lq)
+        with np.errstate(over="ignore"):
+            want = L.tmod(3 * x - 5 * y, lq)
+        np.testing.assert_array_equal(host(T.axpy(dev(x), dev(y), 3, -5, lq)), want)
+        e = np.array([0, 1, -7, 1500])
+        got = host(T.rotate_poly(dev(x), dev(e), lq))
+        for i in range(4):
+            for c in range(2):
+                np.testing.assert_array_equal(got[i, c], L.tmod(O.coef_rotation(x[i, c], e[i]), lq))
+
+
+def _gkey_fft(T, gk):
+    return T.poly_fft(dev(gk))                       # (logN, l, 2, N) -> (logN, l, 2, N/2)
+
+
+@pytest.mark.parametrize("N,log_q,lb,l,sigma,cnt", [(256, 64, 2, 20, 2.0 ** 17, 5),
+                                                    (2048, 64, 2, 20, 2.0 ** 17, 2),
+                                                    (512, 32, 4, 8, 1.0, 4),
+                                                    (128, 64, 8, 8, 2.0 ** 20, 3)])
+def test_trace_pack_vs_exact(cuda_device, T, N, log_q, lb, l, sigma, cnt):
+    rng = np.random.default_rng(N + l)
+    S = rng.integers(0, 2, N)
+    gk = L.galois_key(rng, S, log_q, lb, l, sigma)
+    a, b = L.lwe_encrypt(rng, rng.integers(-8, 8, cnt) << (log_q - 6), S, log_q, sigma)
+    got = host(T.pack_lwe_to_rlwe(dev(a), dev(b), _gkey_fft(T, gk), lb, l, log_q))
+    for i in range(cnt):
+        A, Bp = L.trace_pack(a[i], int(b[i]), gk, lb, l, log_q)
+        if log_q <= 32:
+            assert max_err(got[i, 0], A, log_q) == 0
+            assert max_err(got[i, 1], Bp, log_q) == 0
+        ph_got = L.rlwe_phase(got[i, 0], got[i, 1], S, log_q)
+        ph_want = L.rlwe_phase(A, Bp, S, log_q)
+        assert max_err(ph_got, ph_want, log_q) <= TOL_REL * 2.0 ** log_q
+
+
+@pytest.mark.parametrize("N,log_q,S_,G,nl", [(2048, 64, 2, 8, 4), (256, 64, 1, 3, 4), (512, 32, 3, 2, 2)])
+def test_lut_product_sum_vs_exact(cuda_device, T, N, log_q, S_, G, nl):
+    rng = np.random.default_rng(N * G)
+    rl = L.uniform(rng, (S_, nl, G, 2, N), log_q)
+    lut = rng.integers(-1, 2, (nl, G, N))
+    got = host(T.lut_product_sum(dev(rl), T.poly_fft(dev(lut)), nl, log_q))
+    tol = 0 if log_q <= 32 else TOL_REL * 2.0 ** log_q
+    for s in range(S_):
+        want = L.lut_product_sum(rl[s].reshape(nl * G, 2, N), lut.reshape(nl * G, N), G, nl, log_q)
+        assert max_err(got[s], want, log_q) <= tol
+
+
+@pytest.mark.parametrize("N,n_in,n_lwes,log_q,lb,l,P", [(256, 256, 40, 64, 8, 4, 2),
+                                                        (512, 512, 512, 64, 8, 4, 1),
+                                                        (256, 200, 256, 32, 4, 4, 2)])
+def test_packing_vs_exact(cuda_device, T, N, n_in, n_lwes, log_q, lb, l, P):
+    rng = np.random.default_rng(N + n_lwes)
+    S = rng.integers(0, 2, N)
+    s = rng.integers(0, 2, n_in)
+    ksk = L.packing_ksk(rng, s, S, log_q, lb, l, 2.0 ** 17 if log_q == 64 else 1.0)
+    masks = L.uniform(rng, (P, n_lwes, n_in), log_q)
+    bodies = L.uniform(rng, (P, n_lwes), log_q)
+    kf = T.poly_fft(dev(ksk))                                 # (n_in, l, 2, N/2)
+    got = host(T.packing(dev(masks), dev(bodies), kf, lb, l, log_q))
+    # the transform-domain sum over n_in rows is rounded once; at q = 2^32 the
+    # products are exact up to that single rounding of each coefficient
+    tol = 2.0 if log_q <= 32 else TOL_REL * 2.0 ** log_q
+    for p in range(P):
+        A, Bp = L.packing(masks[p], bodies[p], ksk, lb, l, log_q)
+        assert max_err(got[p, 0], A, log_q) <= tol
+        assert max_err(got[p, 1], Bp, log_q) <= tol
+
+
+def test_packing_decrypts_to_the_lwe_phases(cuda_device, T):
+    rng = np.random.default_rng(9)
+    N, log_q, lb, l = 2048, 64, 8, 4
+    S = rng.integers(0, 2, N)
+    ksk = L.packing_ksk(rng, S, S, log_q, lb, l, 2.0 ** 17)
+    mu = rng.integers(-4, 4, 128) << 56
+    a, b = L.lwe_encrypt(rng, mu, S, log_q, 2.0 ** 17)
+    out = host(T.packing(dev(a[None]), dev(b[None]), T.poly_fft(dev(ksk)), lb, l, log_q))
+    ph = L.rlwe_phase(out[0, 0], out[0, 1], S, log_q)
+    assert np.array_equal(np.rint(ph[:128] / 2.0 ** 56).astype(np.int64), mu >> 56)
+    assert np.abs(ph[128:].astype(np.float64)).max() < 2.0 ** 52
+
+
+@pytest.mark.parametrize("form", ["plain", "batched"])
+def test_blind_rotation_at_q64_vs_exact(cuda_device, form):
+    """The int64 accumulator path at the reference's LSTM modulus 2^64
+    (l = 2, beta = 2^13, c = 3) against the int128 oracle."""
+    from cuTFHE import multiply
+    rng = np.random.default_rng(21)
+    N, n, Lv, lb, c, B = 512, 24, 2, 13, 3, 5
+    s = rng.integers(0, 2, n)
+    S = rng.integers(0, 2, N)
+    bsk_coef = L.bsk_kernel_coef(rng, s, S, 64, lb, Lv, c, 2.0 ** 17)
+    bsk_fft = O.negacyclic_fft(bsk_coef.astype(np.float64))
+    ct = L.uniform(rng, (B, 2, N), 64)
+    idx = rng.integers(0, 2 * N, (B, n // c, 2 ** c)).astype(np.int32)
+    got = host(multiply.mul_bootstrap(dev(ct), dev(bsk_fft), None, dev(idx), lb, 64, Lv, form))
+    want = X.blind_rotate(ct, bsk_coef, idx, lb, 64, Lv)
+    for i in range(B):
+        ph_got = L.rlwe_phase(got[i, 0], got[i, 1], S, 64)
+        ph_want = L.rlwe_phase(want[i, 0], want[i, 1], S, 64)
+        assert max_err(ph_got, ph_want, 64) <= TOL_REL * 2.0 ** 64
+
+
+# ---------------------------------------------------------------------------
+# one encrypted spiking-LSTM timestep
+# ---------------------------------------------------------------------------
+
+class _Model:
+    """A small layer with the reference's structure and the reference's
+    moduli (q = 2^64, beta_x = 2^56): N = 256, n = 48, n_lut = 4, 16 hidden
+    units, 16 inputs.  Every LUT spans pre-activations [-4, 4) in 64 entries."""
+
+    def __init__(self, seed=0):
+        import parameters_crypto as PC
+        self.N, self.n_ks, self.nl, self.hidden, self.input_dim = 256, 48, 4, 16, 16
+        self.params = PC.make_params(degree=self.N, n_ks=self.n_ks, sigma_lwe=2.0 ** 30)
+        self.dp, self.dpl, self.dpk, self.dpp = self.params
+        self.beta_x = 2.0 ** 56
+        self.s = 4.0
+        self.G = self.hidden // self.nl
+        rng = np.random.default_rng(seed)
+        K = self.input_dim + self.hidden
+        # weights in units of 1/16 of a pre-activation: x = (sum_i w_i*v_i + b)/16
+        self.W = {g: rng.integers(-3, 4, (K, self.G)) for g in "fio"}
+        self.b = {g: rng.integers(-8, 9, self.G) for g in "fio"}
+        self.gamma = {g: rng.uniform(-2.0, 2.0, (self.nl, self.G)) for g in "fio"}
+        self.eta = np.ones((self.nl, self.G))
+        self.x = rng.integers(0, 2, (3, self.input_dim))                 # spikes, 3 steps
+
+    def factor(self):
+        """Puts sum_i w_i * (v_i * beta_x) at x * q/(4s) (x = sum w v / 16)."""
+        return 2.0 ** 64 / (4 * self.s) / 16 / self.beta_x
+
+
+def _build_device_layer(m: _Model):
+    import setup_crypto as SC
+    from nn_tfhe.Spike_LSTM import CipherSpikeLSTM
+    sk, ksk_packing, galois_key, bsk, ksk_bs, sk_bsk = SC.sample_crypto_keys(7, m.params, m.beta_x)
+    heavy, _ = SC.get_tfhe_activation(bsk, ksk_bs, ksk_packing, galois_key, np.ones(m.G), m.nl,
+                                      params=m.params, device="cuda:0")
+    lin = [SC.make_linear(m.W[g], m.b[g] * m.beta_x, m.factor(), m.dp, device="cuda:0")
+           for g in "fio"]
+    sv = np.full(m.G, m.s)
+    lut_f, lut_i, lut_o = SC.get_lut(m.nl, m.hidden, m.N, sv, sv, sv, m.gamma["f"],
+                                     m.gamma["i"], m.gamma["o"], m.eta, m.beta_x)
+    lstm = CipherSpikeLSTM(m.dp, m.dpl, m.input_dim, m.hidden, 3, lin[0], lin[1], lin[2], heavy,
+                           m.nl, lut_f, lut_i, lut_o, m.beta_x)
+    return lstm, sk
+
+
+def test_lstm_timesteps_decrypt_to_the_plaintext_rule(cuda_device):
+    """Three timesteps.  At each step the plaintext rule is evaluated on the
+    decrypted previous states (so one unit called near a threshold does not
+    poison later checks); every unit whose gate pre-activations lie at least
+    4 LUT entries from their thresholds (and inside the LUT range) must
+    decrypt exactly to it."""
+    from schemas.RLWE_jax import _phase, encrypt
+    m = _Model()
+    lstm, sk = _build_device_layer(m)
+    rng = np.random.default_rng(3)
+    T_ = m.x.shape[0]
+    msg = np.zeros((T_, m.N), np.int64)
+    msg[:, :m.input_dim] = (m.x * (1 << 56)).astype(np.int64)
+    A, b = encrypt(msg, sk, m.dp, rng)
+    (_, _), (Hs, Cs) = lstm((dev(A), dev(b)), None)
+    entry = 2 * m.s / (m.N // m.nl)
+    H = np.zeros(m.hidden)
+    C = np.zeros(m.hidden)
+    checked = total = 0
+    for t in range(T_):
+        phC = _phase((host(Cs[0][t]), host(Cs[1][t])), sk, m.dp)
+        phH = _phase((host(Hs[0][t]), host(Hs[1][t])), sk, m.dp)
+        decC = np.rint(phC[:m.hidden] / 2.0 ** 56)
+        decH = np.rint(phH[m.input_dim:m.input_dim + m.hidden] / 2.0 ** 56)
+        assert np.abs(np.delete(phC, np.arange(m.hidden))).max() < 2.0 ** 54
+        hx = np.concatenate([m.x[t], H])
+        pre = {g: (hx @ m.W[g] + m.b[g]) / 16.0 for g in "fio"}
+        for k in range(m.nl):
+            for g in range(m.G):
+                u = k * m.G + g
+                total += 1
+                xs = [pre[c][g] for c in "fio"]
+                ds = [abs(pre[c][g] - m.gamma[c][k, g]) for c in "fio"]
+                if max(abs(v) for v in xs) > m.s - 4 * entry or min(ds) < 4 * entry:
+                    continue
+                c_new = C[u] * (pre["f"][g] >= m.gamma["f"][k, g]) + \
+                    m.eta[k, g] * (pre["i"][g] >= m.gamma["i"][k, g])
+                h_new = c_new * (pre["o"][g] >= m.gamma["o"][k, g])
+                assert decC[u] == c_new, (t, u, decC[u], c_new)
+                assert decH[u] == h_new, (t, u, decH[u], h_new)
+                checked += 1
+        H, C = decH, decC
+    assert checked >= total // 4, (checked, total)

@@ -41,7 +41,7 @@ def mult():
 
 def test_library_is_native(cuda_device):
     from cuTFHE import _core
-    assert _core.lib().bslstm_abi_version() == 1
+    assert _core.lib().bslstm_abi_version() == 2
 
 
 def test_rotation_indices_bit_exact(cuda_device, mult):
