@@ -184,6 +184,80 @@ def run_gpu(args):
     return res
 
 
+# one encrypted spiking-LSTM layer at the reference's parameters
+# (parameters_crypto.py: q = 2^64, N = 2048, n = 702, l_bs = 2, Bg = 2^13,
+# c = 3; key switch 2^1 x 20; packing 2^8 x 4; trace packing 2^2 x 20)
+LSTM_CFG = dict(input_dim=128, hidden=128, n_lut=4, seed=0)
+
+
+def run_lstm(args):
+    """Per-timestep latency of CipherSpikeLSTM.step for `batch` sequences:
+    every launch of one step (state extraction, three Linear layers, two LUT
+    preparations, two key switch + many-LUT bootstraps, two packings, the H
+    rotation), inputs resident in HBM.  Returns a dict for the JSON line."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, PKG)
+    import parameters_crypto as PC
+    import setup_crypto as SC
+    from nn_tfhe.Spike_LSTM import CipherSpikeLSTM
+    from schemas.RLWE_jax import encrypt
+
+    cfg = LSTM_CFG
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(cfg["seed"])
+    t0 = time.perf_counter()
+    sk, ksk_packing, galois_key, bsk, ksk_bs, _ = SC.sample_crypto_keys(cfg["seed"])
+    keygen_s = time.perf_counter() - t0
+    hid, nl, K = cfg["hidden"], cfg["n_lut"], cfg["input_dim"] + cfg["hidden"]
+    G = hid // nl
+    heavy, _ = SC.get_tfhe_activation(bsk, ksk_bs, ksk_packing, galois_key, np.ones(G), nl,
+                                      device=dev)
+    # int4 weights U[-7, 7] scaled so that a unit weight moves a pre-activation
+    # by 1/16 of the LUT range (the role of beta_w/s in setup_crypto.py:39-67)
+    factor = PC.q / 16 / 16 / PC.beta_x
+    lin = [SC.make_linear(rng.integers(-7, 8, (K, G)), np.zeros(G), factor, PC.dict_params,
+                          device=dev) for _ in range(3)]
+    s = np.full(G, 4.0)
+    gam = [rng.uniform(-2, 2, (nl, G)) for _ in range(3)]
+    lut_f, lut_i, lut_o = SC.get_lut(nl, hid, PC.degree_lut, s, s, s, gam[0], gam[1], gam[2],
+                                     np.ones((nl, G)))
+    lstm = CipherSpikeLSTM(PC.dict_params, PC.dict_params_lut, cfg["input_dim"], hid, 128, lin[0],
+                           lin[1], lin[2], heavy, nl, lut_f, lut_i, lut_o, PC.beta_x)
+    out = {"workload": f"one encrypted spiking-LSTM layer: input {cfg['input_dim']}, hidden {hid}, "
+                       f"n_lut {nl}, q=2^64, N=2048, n=702, l=2, Bg=2^13, c=3",
+           "keygen_s": round(keygen_s, 2), "bootstraps_per_step_per_sequence": 3 * G}
+    N = PC.degree
+    for S, steps in ((1, args.lstm_steps), (args.lstm_batch, max(4, args.lstm_steps // 4))):
+        if steps <= 0 or S <= 0:
+            continue
+        msg = np.zeros((steps + 2, N), np.int64)
+        msg[:, :cfg["input_dim"]] = rng.integers(0, 2, (steps + 2, cfg["input_dim"])) << 56
+        A, b = encrypt(msg, sk, PC.dict_params, rng)
+        x = torch.stack([torch.as_tensor(A), torch.as_tensor(b)], dim=1).to(dev)   # (T, 2, N)
+        x = x.unsqueeze(1).expand(-1, S, -1, -1).contiguous()                       # (T, S, 2, N)
+        H = torch.zeros(S, 2, N, dtype=torch.int64, device=dev)
+        C = torch.zeros_like(H)
+        for t in range(2):                                                          # warm-up
+            H, C = lstm.step(H, C, x[t])
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(steps)]
+        for t in range(steps):
+            evs[t][0].record(stream)
+            H, C = lstm.step(H, C, x[t + 2])
+            evs[t][1].record(stream)
+        torch.cuda.synchronize()
+        ms = np.array([a.elapsed_time(e) for a, e in evs])
+        key = "latency_batch1" if S == 1 else f"throughput_batch{S}"
+        out[key] = {"sequences": S, "steps": steps, "p50_ms": round(float(np.percentile(ms, 50)), 3),
+                    "p99_ms": round(float(np.percentile(ms, 99)), 3),
+                    "mean_ms": round(float(ms.mean()), 3),
+                    "bootstraps_per_s": round(3 * G * S / (ms.mean() / 1e3), 1)}
+    return out
+
+
 def cpu_baseline_child(sample: int):
     """Oracle (NumPy FP64 restatement of the reference algorithm) on a
     bounded sample of the same workload, single-threaded."""
@@ -239,6 +313,10 @@ def main():
                     help="also run one elementwise negation over this many MiB (reads and "
                          "writes that many bytes): the calibration kernel for HBM counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lstm-steps", type=int, default=32,
+                    help="timesteps of the encrypted spiking-LSTM layer to time (0: skip)")
+    ap.add_argument("--lstm-batch", type=int, default=32,
+                    help="sequences per timestep for the LSTM throughput line")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
     args = ap.parse_args()
     if args.cpu_baseline_child:
@@ -248,7 +326,9 @@ def main():
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the engine has no "
               f"multi-GPU path", file=sys.stderr)
         sys.exit(2)
-    res = run_gpu(args)
+    res = run_gpu(args) if args.steps > 0 else {}
+    if args.lstm_steps > 0:
+        res["lstm"] = run_lstm(args)
     if args.copy_mib:
         import torch
         src = torch.ones(args.copy_mib * 2 ** 18, dtype=torch.float32, device="cuda:0")

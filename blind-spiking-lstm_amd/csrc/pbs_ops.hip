@@ -239,12 +239,46 @@ __global__ void rotate_extract_i64_kernel(const int64_t *__restrict__ br,
 }
 
 // exact int64 key switch: a' = sum d*KSK_a, b' = b + sum d*KSK_b mod 2^log2_q,
-// digits by the float rule restated on integers (polynomial_jax.py:90-111)
+// digits by the float rule restated on integers (polynomial_jax.py:90-111).
+// The reduction over the n_in inputs is split across gridDim.z workgroups
+// (chunks of ic_max inputs, round robin): at LSTM batch sizes (tens of
+// ciphertexts) the output tile alone gives only a handful of workgroups.
+// Partial sums meet in the outputs through 64-bit integer atomics, which
+// are exact and order-independent, so the result is bit-reproducible.
+// key_switch_i64_init writes a' = 0, b' = b first; key_switch_i64_final
+// reduces to the centred representative afterwards.
+__global__ void key_switch_i64_init(const int64_t *__restrict__ b_in, int batch, int n_out,
+                                    int64_t *__restrict__ a_out, int64_t *__restrict__ b_out)
+{
+    const int64_t total = (int64_t)batch * (n_out + 1);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / (n_out + 1);
+        const int j = (int)(e % (n_out + 1));
+        if (j < n_out)
+            a_out[row * n_out + j] = 0;
+        else
+            b_out[row] = b_in[row];
+    }
+}
+
+__global__ void key_switch_i64_final(int batch, int n_out, int lq, int64_t *__restrict__ a_out,
+                                     int64_t *__restrict__ b_out)
+{
+    const int64_t total = (int64_t)batch * (n_out + 1);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / (n_out + 1);
+        const int j = (int)(e % (n_out + 1));
+        int64_t *p = j < n_out ? a_out + row * n_out + j : b_out + row;
+        *p = tmod64((uint64_t)*p, lq);
+    }
+}
+
 __global__ __launch_bounds__(KS_THREADS) void key_switch_i64_kernel(
-    const int64_t *__restrict__ a_in, const int64_t *__restrict__ b_in,
-    const int64_t *__restrict__ ksk_a, const int64_t *__restrict__ ksk_b, int batch, int n_in,
-    int n_out, int lb, int l_ks, int ic_max, int lq, int64_t *__restrict__ a_out,
-    int64_t *__restrict__ b_out)
+    const int64_t *__restrict__ a_in, const int64_t *__restrict__ ksk_a,
+    const int64_t *__restrict__ ksk_b, int batch, int n_in, int n_out, int lb, int l_ks,
+    int ic_max, int lq, int64_t *__restrict__ a_out, int64_t *__restrict__ b_out)
 {
     extern __shared__ int32_t digi[];   // [l_ks][KS_RB][ic_max]
     const int j = blockIdx.x * KS_THREADS + threadIdx.x;
@@ -254,7 +288,7 @@ __global__ __launch_bounds__(KS_THREADS) void key_switch_i64_kernel(
 #pragma unroll
     for (int r = 0; r < KS_RB; r++)
         acc[r] = 0;
-    for (int i0 = 0; i0 < n_in; i0 += ic_max) {
+    for (int i0 = blockIdx.z * ic_max; i0 < n_in; i0 += gridDim.z * ic_max) {
         __syncthreads();
         for (int e = threadIdx.x; e < KS_RB * ic_max; e += KS_THREADS) {
             const int r = e / ic_max, ii = e % ic_max;
@@ -295,10 +329,9 @@ __global__ __launch_bounds__(KS_THREADS) void key_switch_i64_kernel(
         const int row = r0 + r;
         if (row >= batch)
             break;
-        if (j < n_out)
-            a_out[(size_t)row * n_out + j] = tmod64(acc[r], lq);
-        else
-            b_out[row] = tmod64((uint64_t)b_in[row] + acc[r], lq);
+        unsigned long long *dst = reinterpret_cast<unsigned long long *>(
+            j < n_out ? a_out + (size_t)row * n_out + j : b_out + row);
+        atomicAdd(dst, (unsigned long long)acc[r]);
     }
 }
 
@@ -458,14 +491,26 @@ extern "C" int bslstm_key_switch_lwe_i64(void *stream, const int64_t *a_in, cons
         return fail(BSLSTM_INVALID_ARGUMENT, "bad shapes");
     if (batch == 0)
         return BSLSTM_OK;
-    if (!a_in || !b_in || !ksk_a || !ksk_b || !a_out || !b_out)
-        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
-    dim3 grid((unsigned)((n_out + 1 + KS_THREADS - 1) / KS_THREADS),
-              (unsigned)((batch + KS_RB - 1) / KS_RB));
+    if (!a_in || !b_in || !ksk_a || !ksk_b || !a_out || !b_out || b_out == b_in || a_out == a_in)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL or aliased operand");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t total = batch * (n_out + 1);
+    const unsigned eb = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(key_switch_i64_init, dim3(eb), dim3(256), 0, s, b_in, (int)batch,
+                       (int)n_out, a_out, b_out);
     const int ic_max = (int)std::min<int64_t>(32, (2 * KS_LDS_DOUBLES) / (l_ks * KS_RB));
+    const int64_t gx = (n_out + 1 + KS_THREADS - 1) / KS_THREADS, gy = (batch + KS_RB - 1) / KS_RB;
+    const int64_t chunks = (n_in + ic_max - 1) / ic_max;
+    // enough workgroups to fill the chip (~2 per CU) when the tile grid is small
+    const int64_t gz = std::max<int64_t>(1, std::min<int64_t>(chunks, 512 / (gx * gy)));
     const size_t lds = sizeof(int32_t) * (size_t)l_ks * KS_RB * ic_max;
-    hipLaunchKernelGGL(key_switch_i64_kernel, grid, dim3(KS_THREADS), lds, (hipStream_t)stream,
-                       a_in, b_in, ksk_a, ksk_b, (int)batch, (int)n_in, (int)n_out,
-                       (int)log2_beta_ks, (int)l_ks, ic_max, (int)log2_q, a_out, b_out);
+    hipLaunchKernelGGL(key_switch_i64_kernel, dim3((unsigned)gx, (unsigned)gy, (unsigned)gz),
+                       dim3(KS_THREADS), lds, s, a_in, ksk_a, ksk_b, (int)batch, (int)n_in,
+                       (int)n_out, (int)log2_beta_ks, (int)l_ks, ic_max, (int)log2_q, a_out, b_out);
+    if (int st = launch_status("key switch"))
+        return st;
+    if (log2_q < 64)
+        hipLaunchKernelGGL(key_switch_i64_final, dim3(eb), dim3(256), 0, s, (int)batch, (int)n_out,
+                           (int)log2_q, a_out, b_out);
     return launch_status("key switch");
 }
