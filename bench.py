@@ -258,6 +258,91 @@ def run_lstm(args):
     return out
 
 
+# full classifier: 32 sequences of lengths U[8, 64], Zipf(1.1) token ids over
+# a 30522-word vocabulary, embedding 128 binarized to spikes, hidden 256,
+# n_lut 4, head Linear(256 -> 32) -> ReLU bootstrap -> packing -> Linear(32 -> 2)
+CLF_CFG = dict(sequences=32, len_lo=8, len_hi=64, vocab=30522, zipf=1.1, input_dim=128, hidden=256,
+               n_lut=4, head_hidden=32, classes=2, seed=1)
+
+
+def run_classifier(args):
+    """All timesteps of 32 uneven-length sequences through the timestep engine
+    (live sequences batched per step, steps replayed from captured HIP
+    graphs), then the encrypted head; the timed region starts with every
+    input ciphertext resident in HBM."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, PKG)
+    import parameters_crypto as PC
+    import setup_crypto as SC
+    from nn_tfhe.Spike_LSTM import CipherMLP, CipherSpikeLSTM
+    from nn_tfhe.engine import SequenceEngine
+    from cuTFHE.multiply import tuned_forms
+    from schemas.RLWE_jax import encrypt
+    from schemas.text_jax import rotate_ciphertext
+
+    cfg = CLF_CFG
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(cfg["seed"])
+    sk, ksk_packing, galois_key, bsk, ksk_bs, _ = SC.sample_crypto_keys(cfg["seed"])
+    hid, nl, K = cfg["hidden"], cfg["n_lut"], cfg["input_dim"] + cfg["hidden"]
+    G = hid // nl
+    heavy, relu = SC.get_tfhe_activation(bsk, ksk_bs, ksk_packing, galois_key,
+                                         np.full(cfg["head_hidden"], 4.0), nl, device=dev)
+    factor = PC.q / 16 / 16 / PC.beta_x
+    lin = [SC.make_linear(rng.integers(-7, 8, (K, G)), np.zeros(G), factor, PC.dict_params,
+                          device=dev) for _ in range(3)]
+    s4 = np.full(G, 4.0)
+    gam = [rng.uniform(-2, 2, (nl, G)) for _ in range(3)]
+    lut_f, lut_i, lut_o = SC.get_lut(nl, hid, PC.degree_lut, s4, s4, s4, gam[0], gam[1], gam[2],
+                                     np.ones((nl, G)))
+    lstm = CipherSpikeLSTM(PC.dict_params, PC.dict_params_lut, cfg["input_dim"], hid, cfg["len_hi"],
+                           lin[0], lin[1], lin[2], heavy, nl, lut_f, lut_i, lut_o, PC.beta_x)
+    # int8 head weights
+    head1 = SC.make_linear(rng.integers(-127, 128, (hid, cfg["head_hidden"])),
+                           np.zeros(cfg["head_hidden"]), 1.0, PC.dict_params, device=dev)
+    head2 = SC.make_linear(rng.integers(-127, 128, (cfg["head_hidden"], cfg["classes"])),
+                           np.zeros(cfg["classes"]), 1.0, PC.dict_params, device=dev)
+    head = CipherMLP(PC.dict_params, PC.dict_params_lut, hid, head1, head2, relu)
+    # client side: token ids, binarized embeddings, encryption
+    emb = rng.standard_normal((cfg["vocab"], cfg["input_dim"])) > 0
+    lens = rng.integers(cfg["len_lo"], cfg["len_hi"] + 1, cfg["sequences"])
+    seqs = []
+    for L_ in lens:
+        tok = np.minimum(rng.zipf(cfg["zipf"], L_), cfg["vocab"]) - 1
+        msg = np.zeros((L_, PC.degree), np.int64)
+        msg[:, :cfg["input_dim"]] = emb[tok].astype(np.int64) << 56
+        A, b = encrypt(msg, sk, PC.dict_params, rng)
+        seqs.append((torch.as_tensor(A).to(dev), torch.as_tensor(b).to(dev)))
+    shift = torch.full((1,), -cfg["input_dim"], dtype=torch.int64, device=dev)
+    out = {"workload": f"encrypted spiking-LSTM classifier: {cfg['sequences']} sequences, lengths "
+                       f"U[{cfg['len_lo']},{cfg['len_hi']}], Zipf({cfg['zipf']}) tokens over "
+                       f"{cfg['vocab']}, embedding {cfg['input_dim']} -> spikes, hidden {hid}, "
+                       f"n_lut {nl}, head {hid}->{cfg['head_hidden']}->ReLU->{cfg['classes']}, "
+                       f"q=2^64, N=2048",
+           "total_timesteps": int(lens.sum()),
+           "bootstraps": int(lens.sum()) * 3 * G + cfg["sequences"] * cfg["head_hidden"]}
+    for use_graph in (False, True):
+        eng = SequenceEngine(lstm, use_graph=use_graph)
+        eng.run(seqs)                                     # warm-up: tuning, graph capture
+        torch.cuda.synchronize()
+        steps: list = []
+        t0 = time.perf_counter()
+        H, C = eng.run(seqs, step_times=steps)
+        Hh = rotate_ciphertext((H[:, 0], H[:, 1]), shift)
+        logits = head(Hh)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = np.array([m for _, m in steps])
+        out["graph" if use_graph else "eager"] = {
+            "wall_s": round(wall, 3), "bootstraps_per_s": round(out["bootstraps"] / wall, 1),
+            "step_p50_ms": round(float(np.percentile(ms, 50)), 3),
+            "step_p99_ms": round(float(np.percentile(ms, 99)), 3), "steps": len(ms)}
+    out["logits_shape"] = list(logits[0].shape)
+    out["tuned_forms"] = {str(k): v for k, v in tuned_forms().items()}
+    return out
+
+
 def cpu_baseline_child(sample: int):
     """Oracle (NumPy FP64 restatement of the reference algorithm) on a
     bounded sample of the same workload, single-threaded."""
@@ -317,6 +402,8 @@ def main():
                     help="timesteps of the encrypted spiking-LSTM layer to time (0: skip)")
     ap.add_argument("--lstm-batch", type=int, default=32,
                     help="sequences per timestep for the LSTM throughput line")
+    ap.add_argument("--classifier", type=int, default=1,
+                    help="run the 32-sequence encrypted classifier (0: skip)")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
     args = ap.parse_args()
     if args.cpu_baseline_child:
@@ -329,6 +416,8 @@ def main():
     res = run_gpu(args) if args.steps > 0 else {}
     if args.lstm_steps > 0:
         res["lstm"] = run_lstm(args)
+    if args.classifier:
+        res["classifier"] = run_classifier(args)
     if args.copy_mib:
         import torch
         src = torch.ones(args.copy_mib * 2 ** 18, dtype=torch.float32, device="cuda:0")

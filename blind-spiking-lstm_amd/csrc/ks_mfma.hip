@@ -1,0 +1,280 @@
+// LWE key switch as exact int8 MFMA GEMMs (gfx950 v_mfma_i32_32x32x32_i8).
+//
+// The key switch of schemas/LWE_jax.py:97-117 is the product of the digit
+// matrix D (batch x K, K = n_in*l, signed digits of the float rule of
+// polynomial_jax.py:90-111, |d| <= beta/2 <= 64) with the key matrix
+// (K x (n_out+1), the body key as the last column), mod 2^64.  The key is cut
+// into eight balanced signed bytes, key = sum_b k_b 2^(8b) mod 2^64 with
+// k_b in [-128, 127]; every limb product D x k_b is an int8 GEMM whose int32
+// sums are exact (|sum| <= K * 64 * 128 < 2^31 for K < 2^18), and
+// sum_b (int64)acc_b << 8b mod 2^64 is the exact product.  This is the
+// "split" form: integer-exact, on the matrix cores, at any modulus <= 2^64.
+//
+// Layouts (chosen for 16-byte loads per lane, one 1 KiB block per wave and
+// MFMA step):
+//   k = i*l + lev (input-major, level-minor)
+//   key tiles  [kt][nt][b][lane*16 + j]: limb b of key row k = kt*32 +
+//              (lane/32)*16 + j, column n = nt*32 + lane%32
+//   digit tiles [mt][kt][lane*16 + j]:  digit of row m = mt*32 + lane%32,
+//              k = kt*32 + (lane/32)*16 + j
+// Lane l of both MFMA operands carries the same 16 k values, so the product
+// does not depend on the instruction's internal k order.  The 32x32 int32
+// result sits in the usual layout: register r of lane l is row
+// 8*(r/4) + 4*(l/32) + r%4, column l%32.
+//
+// The reduction over K is split across gridDim.z workgroups when the output
+// alone gives too few; partial int64 results meet in the outputs through
+// exact 64-bit integer atomics (order-independent, bit-reproducible).
+#include <algorithm>
+
+#include "capi_internal.h"
+#include "torus_device.h"
+
+namespace bslstm {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int KM_WAVES = 4;   // waves per workgroup, one 32-column tile each
+
+__global__ void ks_pack_key_kernel(const int64_t *__restrict__ ksk_a,
+                                   const int64_t *__restrict__ ksk_b, int l, int n_in, int n_out,
+                                   int KT, int NT, int8_t *__restrict__ tiles)
+{
+    // one thread per (kt, nt, lane): 16 k values x 8 limbs
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)KT * NT * 64;
+    if (gid >= total)
+        return;
+    const int lane = (int)(gid % 64);
+    const int64_t blk = gid / 64;
+    const int nt = (int)(blk % NT);
+    const int kt = (int)(blk / NT);
+    const int n = nt * 32 + lane % 32;
+    const int K = n_in * l;
+    int8_t limbs[8][16];
+    for (int j = 0; j < 16; j++) {
+        const int k = kt * 32 + (lane / 32) * 16 + j;
+        uint64_t v = 0;
+        if (k < K && n <= n_out) {
+            const int i = k / l, lev = k % l;
+            v = (uint64_t)(n < n_out ? ksk_a[((size_t)lev * n_in + i) * n_out + n]
+                                     : ksk_b[(size_t)lev * n_in + i]);
+        }
+        int carry = 0;
+        for (int b = 0; b < 8; b++) {
+            int d = (int)((v >> (8 * b)) & 255) + carry;
+            carry = d >= 128;
+            limbs[b][j] = (int8_t)(d - carry * 256);
+        }
+    }
+    for (int b = 0; b < 8; b++) {
+        v4i w;
+        for (int q = 0; q < 4; q++)
+            w[q] = (int)((uint32_t)(uint8_t)limbs[b][4 * q] | ((uint32_t)(uint8_t)limbs[b][4 * q + 1] << 8) |
+                         ((uint32_t)(uint8_t)limbs[b][4 * q + 2] << 16) |
+                         ((uint32_t)(uint8_t)limbs[b][4 * q + 3] << 24));
+        *reinterpret_cast<v4i *>(tiles + (((size_t)kt * NT + nt) * 8 + b) * 1024 + lane * 16) = w;
+    }
+}
+
+__global__ void ks_digits_kernel(const int64_t *__restrict__ a_in, int batch, int n_in, int l,
+                                 int lb, int lq, int KT, int MT, int8_t *__restrict__ tiles)
+{
+    // one thread per (mt, kt, lane): 16 consecutive k of one row
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)MT * KT * 64;
+    if (gid >= total)
+        return;
+    const int lane = (int)(gid % 64);
+    const int64_t blk = gid / 64;
+    const int kt = (int)(blk % KT);
+    const int mt = (int)(blk / KT);
+    const int row = mt * 32 + lane % 32;
+    const int K = n_in * l;
+    const int k0 = kt * 32 + (lane / 32) * 16;
+    int8_t dg[16];
+    int cur_i = -1;
+    int64_t r = 0;
+    int next_lev = 0;
+    for (int j = 0; j < 16; j++) {
+        const int k = k0 + j;
+        int d = 0;
+        if (row < batch && k < K) {
+            const int i = k / l, lev = k % l;
+            if (i != cur_i) {   // start the digit recursion of a new input
+                cur_i = i;
+                r = tmod((uint64_t)a_in[(size_t)row * n_in + i], lq);
+                next_lev = 0;
+            }
+            int64_t dd = 0;
+            while (next_lev <= lev)   // levels arrive in order within one input
+                dd = next_digit(r, lq - (++next_lev) * lb);
+            d = (int)dd;
+        }
+        dg[j] = (int8_t)d;
+    }
+    v4i w;
+    for (int q = 0; q < 4; q++)
+        w[q] = (int)((uint32_t)(uint8_t)dg[4 * q] | ((uint32_t)(uint8_t)dg[4 * q + 1] << 8) |
+                     ((uint32_t)(uint8_t)dg[4 * q + 2] << 16) | ((uint32_t)(uint8_t)dg[4 * q + 3] << 24));
+    *reinterpret_cast<v4i *>(tiles + ((size_t)mt * KT + kt) * 1024 + lane * 16) = w;
+}
+
+__global__ void ks_out_init_kernel(const int64_t *__restrict__ b_in, int batch, int n_out,
+                                   int64_t *__restrict__ a_out, int64_t *__restrict__ b_out)
+{
+    const int64_t total = (int64_t)batch * (n_out + 1);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / (n_out + 1);
+        const int j = (int)(e % (n_out + 1));
+        if (j < n_out)
+            a_out[row * n_out + j] = 0;
+        else
+            b_out[row] = b_in[row];
+    }
+}
+
+__global__ void ks_out_final_kernel(int batch, int n_out, int lq, int64_t *__restrict__ a_out,
+                                    int64_t *__restrict__ b_out)
+{
+    const int64_t total = (int64_t)batch * (n_out + 1);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / (n_out + 1);
+        const int j = (int)(e % (n_out + 1));
+        int64_t *p = j < n_out ? a_out + row * n_out + j : b_out + row;
+        *p = tmod((uint64_t)*p, lq);
+    }
+}
+
+__global__ __launch_bounds__(64 * KM_WAVES) void ks_mfma_kernel(
+    const int8_t *__restrict__ dtiles, const int8_t *__restrict__ ktiles, int batch, int n_out,
+    int KT, int NT, int kt_per_split, int64_t *__restrict__ a_out, int64_t *__restrict__ b_out)
+{
+    const int lane = threadIdx.x & 63;
+    const int nt = blockIdx.x * KM_WAVES + (threadIdx.x >> 6);
+    const int mt = blockIdx.y;
+    const int kt0 = blockIdx.z * kt_per_split;
+    const int kt1 = min(KT, kt0 + kt_per_split);
+    if (nt >= NT)
+        return;
+    v16i acc[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+        acc[b] = (v16i){0};
+    const int8_t *ap = dtiles + ((size_t)mt * KT) * 1024 + lane * 16;
+    const int8_t *bp = ktiles + lane * 16;
+#pragma unroll 1
+    for (int kt = kt0; kt < kt1; kt++) {
+        const v4i a = *reinterpret_cast<const v4i *>(ap + (size_t)kt * 1024);
+        const int8_t *bk = bp + ((size_t)kt * NT + nt) * 8 * 1024;
+        v4i bb[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            bb[b] = *reinterpret_cast<const v4i *>(bk + b * 1024);
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bb[b], acc[b], 0, 0, 0);
+    }
+    const int n = nt * 32 + (lane & 31);
+    if (n > n_out)
+        return;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = mt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (m >= batch)
+            continue;
+        uint64_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            v += (uint64_t)(int64_t)acc[b][r] << (8 * b);
+        unsigned long long *dst = reinterpret_cast<unsigned long long *>(
+            n < n_out ? a_out + (size_t)m * n_out + n : b_out + m);
+        atomicAdd(dst, (unsigned long long)v);
+    }
+}
+
+static int tiles_k(int64_t n_in, int64_t l) { return (int)((n_in * l + 31) / 32); }
+static int tiles_n(int64_t n_out) { return (int)((n_out + 1 + 31) / 32); }
+
+}  // namespace bslstm
+
+using namespace bslstm;
+
+extern "C" int64_t bslstm_ks_mfma_key_bytes(int64_t l, int64_t n_in, int64_t n_out)
+{
+    if (l < 1 || n_in < 1 || n_out < 1 || n_in * l >= (1 << 18))
+        return -1;
+    return (int64_t)tiles_k(n_in, l) * tiles_n(n_out) * 8 * 1024;
+}
+
+extern "C" int bslstm_ks_mfma_prepare_key(void *stream, const int64_t *ksk_a, const int64_t *ksk_b,
+                                          int64_t l, int64_t n_in, int64_t n_out, int8_t *key_tiles)
+{
+    clear_error();
+    if (bslstm_ks_mfma_key_bytes(l, n_in, n_out) < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad l/n_in/n_out (n_in*l < 2^18)");
+    if (!ksk_a || !ksk_b || !key_tiles)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const int KT = tiles_k(n_in, l), NT = tiles_n(n_out);
+    const int64_t total = (int64_t)KT * NT * 64;
+    hipLaunchKernelGGL(ks_pack_key_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, ksk_a, ksk_b, (int)l, (int)n_in, (int)n_out, KT, NT,
+                       key_tiles);
+    return launch_status("key-switch key tiles");
+}
+
+extern "C" int64_t bslstm_ks_mfma_workspace_bytes(int64_t batch, int64_t l, int64_t n_in)
+{
+    if (batch < 0 || l < 1 || n_in < 1)
+        return -1;
+    return ((batch + 31) / 32) * (int64_t)tiles_k(n_in, l) * 1024;
+}
+
+extern "C" int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, const int64_t *b_in,
+                                          const int8_t *key_tiles, int64_t batch, int64_t n_in,
+                                          int64_t n_out, int64_t log2_beta_ks, int64_t l_ks,
+                                          int64_t log2_q, void *workspace, int64_t workspace_bytes,
+                                          int64_t *a_out, int64_t *b_out)
+{
+    clear_error();
+    if (log2_q < 2 || log2_q > 64)
+        return fail(BSLSTM_INVALID_ARGUMENT, "log2_q must be in [2, 64]");
+    if (l_ks < 1 || log2_beta_ks < 1 || log2_beta_ks > 7 || l_ks * log2_beta_ks > log2_q)
+        return fail(BSLSTM_INVALID_ARGUMENT,
+                    "int8 digits need 1 <= log2_beta_ks <= 7 and l_ks*log2_beta_ks <= log2_q");
+    if (bslstm_ks_mfma_key_bytes(l_ks, n_in, n_out) < 0 || batch < 0 || batch > (1LL << 24))
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad shapes (n_in*l_ks < 2^18)");
+    if (batch == 0)
+        return BSLSTM_OK;
+    if (workspace_bytes < bslstm_ks_mfma_workspace_bytes(batch, l_ks, n_in))
+        return fail(BSLSTM_INVALID_ARGUMENT, "workspace too small");
+    if (!a_in || !b_in || !key_tiles || !workspace || !a_out || !b_out || b_out == b_in ||
+        a_out == a_in)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL or aliased operand");
+    hipStream_t s = (hipStream_t)stream;
+    const int KT = tiles_k(n_in, l_ks), NT = tiles_n(n_out), MT = (int)((batch + 31) / 32);
+    int8_t *dt = reinterpret_cast<int8_t *>(workspace);
+    const int64_t dtot = (int64_t)MT * KT * 64;
+    hipLaunchKernelGGL(ks_digits_kernel, dim3((unsigned)((dtot + 255) / 256)), dim3(256), 0, s, a_in,
+                       (int)batch, (int)n_in, (int)l_ks, (int)log2_beta_ks, (int)log2_q, KT, MT, dt);
+    const int64_t total = batch * (n_out + 1);
+    const unsigned eb = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(ks_out_init_kernel, dim3(eb), dim3(256), 0, s, b_in, (int)batch, (int)n_out,
+                       a_out, b_out);
+    const int gx = (NT + KM_WAVES - 1) / KM_WAVES;
+    // split K until the grid has ~2 workgroups per CU
+    int ks = (int)std::max<int64_t>(1, std::min<int64_t>(KT, 512 / ((int64_t)gx * MT)));
+    const int kps = (KT + ks - 1) / ks;
+    ks = (KT + kps - 1) / kps;
+    hipLaunchKernelGGL(ks_mfma_kernel, dim3(gx, MT, ks), dim3(64 * KM_WAVES), 0, s, dt, key_tiles,
+                       (int)batch, (int)n_out, KT, NT, kps, a_out, b_out);
+    if (int st = launch_status("key switch (MFMA)"))
+        return st;
+    if (log2_q < 64)
+        hipLaunchKernelGGL(ks_out_final_kernel, dim3(eb), dim3(256), 0, s, (int)batch, (int)n_out,
+                           (int)log2_q, a_out, b_out);
+    return launch_status("key switch (MFMA)");
+}

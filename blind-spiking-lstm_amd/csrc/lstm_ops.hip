@@ -27,50 +27,9 @@
 
 #include "capi_internal.h"
 #include "fold_fft.h"
+#include "torus_device.h"
 
 namespace bslstm {
-
-// centred representative mod 2^lq of a 64-bit class
-__device__ __forceinline__ int64_t tmod(uint64_t x, int lq)
-{
-    const int s = 64 - lq;
-    return (int64_t)(x << s) >> s;
-}
-
-// round_half_even(r / 2^s) for s in [1, 63], without overflow
-__device__ __forceinline__ int64_t rhe_shift(int64_t r, int s)
-{
-    const int64_t fl = r >> s;
-    const uint64_t rem = (uint64_t)r & ((1ULL << s) - 1);
-    const uint64_t half = 1ULL << (s - 1);
-    return fl + (int64_t)((rem > half) | ((rem == half) & ((uint64_t)fl & 1)));
-}
-
-// Next signed digit of the reference's float decomposition
-// (polynomial_jax.py:90-111: x = gamma/q; d = round(x*beta); x = x*beta - d)
-// restated exactly on the centred integer residual r: d = round_half_even(
-// r / 2^s), r -= d*2^s, with s = log2_q - level*log2_beta (levels from 1).
-__device__ __forceinline__ int64_t next_digit(int64_t &r, int s)
-{
-    if (s <= 0) {
-        const int64_t d = r;
-        r = 0;
-        return d;
-    }
-    const int64_t d = rhe_shift(r, s);
-    r = (int64_t)((uint64_t)r - ((uint64_t)d << s));
-    return d;
-}
-
-// Torus class of a rounded FP64 transform output (jax_inversefourier's
-// round, then centred_mod q).
-__device__ __forceinline__ int64_t f64_to_torus(double v, double q, double inv_q, int lq)
-{
-    v = rint(v - q * rint(v * inv_q));
-    if (lq >= 64)
-        return v >= 0x1p63 ? INT64_MIN : (int64_t)v;
-    return tmod((uint64_t)(int64_t)v, lq);
-}
 
 template <int N>
 __device__ __forceinline__ void forward_any(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)

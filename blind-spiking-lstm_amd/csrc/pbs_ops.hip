@@ -5,6 +5,7 @@
 #include <algorithm>
 
 #include "capi_internal.h"
+#include "torus_device.h"
 
 namespace bslstm {
 
@@ -159,20 +160,6 @@ __global__ __launch_bounds__(KS_THREADS) void key_switch_kernel(
 // parameters, parameters_crypto.py:6): float64 cannot hold those classes, so
 // ciphertexts are int64 centred representatives and sums wrap mod 2^64.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int64_t tmod64(uint64_t x, int lq)
-{
-    const int s = 64 - lq;
-    return (int64_t)(x << s) >> s;
-}
-
-__device__ __forceinline__ int64_t rhe_shift64(int64_t r, int s)
-{
-    const int64_t fl = r >> s;
-    const uint64_t rem = (uint64_t)r & ((1ULL << s) - 1);
-    const uint64_t half = 1ULL << (s - 1);
-    return fl + (int64_t)((rem > half) | ((rem == half) & ((uint64_t)fl & 1)));
-}
-
 // rotation exponents from int64 masks: the group sum is formed exactly
 // (mod 2^64, then centred mod q), then the reference's FP64 rounding
 // round(2N*val/(q*n_lut))*n_lut mod 2N (multiply.py:67-79).
@@ -193,7 +180,7 @@ __global__ void rotation_indices_i64_kernel(const int64_t *__restrict__ pk, int6
     for (int j = 0; j < c; j++)
         if ((p >> (c - 1 - j)) & 1)
             acc += (uint64_t)row[j];
-    const double val = (double)tmod64(acc, lq);
+    const double val = (double)tmod(acc, lq);
     const int32_t v = (int32_t)(rint((two_n * val) / q_nlut) * n_lut);
     int32_t r = v % two_N;
     if (r < 0)
@@ -227,14 +214,14 @@ __global__ void rotate_extract_i64_kernel(const int64_t *__restrict__ br,
         uint64_t v = (uint64_t)(w < N ? A[w] : A[w - N]);
         if ((w >= N) != neg)
             v = 0ULL - v;
-        masks[((size_t)i * n_ext + k) * N + j] = tmod64(v, lq);
+        masks[((size_t)i * n_ext + k) * N + j] = tmod(v, lq);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         int w = (k - alpha) % twoN;
         if (w < 0)
             w += twoN;
         const uint64_t v = w < N ? (uint64_t)Bp[w] : 0ULL - (uint64_t)Bp[w - N];
-        bodies[(size_t)i * n_ext + k] = tmod64(v, lq);
+        bodies[(size_t)i * n_ext + k] = tmod(v, lq);
     }
 }
 
@@ -271,7 +258,7 @@ __global__ void key_switch_i64_final(int batch, int n_out, int lq, int64_t *__re
         const int64_t row = e / (n_out + 1);
         const int j = (int)(e % (n_out + 1));
         int64_t *p = j < n_out ? a_out + row * n_out + j : b_out + row;
-        *p = tmod64((uint64_t)*p, lq);
+        *p = tmod((uint64_t)*p, lq);
     }
 }
 
@@ -293,7 +280,7 @@ __global__ __launch_bounds__(KS_THREADS) void key_switch_i64_kernel(
         for (int e = threadIdx.x; e < KS_RB * ic_max; e += KS_THREADS) {
             const int r = e / ic_max, ii = e % ic_max;
             const int row = r0 + r, i = i0 + ii;
-            int64_t x = (row < batch && i < n_in) ? tmod64((uint64_t)a_in[(size_t)row * n_in + i], lq) : 0;
+            int64_t x = (row < batch && i < n_in) ? tmod((uint64_t)a_in[(size_t)row * n_in + i], lq) : 0;
             for (int lev = 0; lev < l_ks; lev++) {
                 const int sh = lq - (lev + 1) * lb;
                 int64_t d;
@@ -301,7 +288,7 @@ __global__ __launch_bounds__(KS_THREADS) void key_switch_i64_kernel(
                     d = x;
                     x = 0;
                 } else {
-                    d = rhe_shift64(x, sh);
+                    d = rhe_shift(x, sh);
                     x = (int64_t)((uint64_t)x - ((uint64_t)d << sh));
                 }
                 digi[(lev * KS_RB + r) * ic_max + ii] = (int32_t)d;

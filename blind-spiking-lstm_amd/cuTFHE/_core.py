@@ -52,6 +52,11 @@ SIGNATURES = {
                        ctypes.c_int),
     "bslstm_torus_axpy": ([_P, _P, _P, _I64, _I64, _I64, _I64, _P], ctypes.c_int),
     "bslstm_rotate_poly": ([_P, _P, _I64, _I64, _P, _I64, _I64, _P], ctypes.c_int),
+    "bslstm_ks_mfma_key_bytes": ([_I64, _I64, _I64], ctypes.c_int64),
+    "bslstm_ks_mfma_prepare_key": ([_P, _P, _P, _I64, _I64, _I64, _P], ctypes.c_int),
+    "bslstm_ks_mfma_workspace_bytes": ([_I64, _I64, _I64], ctypes.c_int64),
+    "bslstm_key_switch_lwe_mfma": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64,
+                                    _P, _P], ctypes.c_int),
 }
 
 

@@ -57,6 +57,20 @@ def prepare_bsk(BSK: torch.Tensor, n: int, collapse: int) -> torch.Tensor:
 
 FORMS = ("plain", "batched")
 
+# Launch-shape autotuner: the faster form per (degree, l, n_iter, m, log2
+# batch bucket), measured once on the live call (both forms run and give
+# bit-identical results, so the measurement costs one extra launch).
+_TUNED: dict = {}
+
+
+def _bucket(batch: int) -> int:
+    return max(0, int(batch - 1).bit_length())
+
+
+def tuned_forms() -> dict:
+    """The autotuner's choices so far: {(degree, l, n_iter, m, bucket): form}."""
+    return dict(_TUNED)
+
 
 def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft: torch.Tensor | None,
                   indices: torch.Tensor, log_beta: int, log_q: int, l: int,
@@ -65,10 +79,33 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     csrc/mul_bootstrap.cuh:192-243): int64 ct (…,2,N) -> int64 (batch,2,N).
 
     form "plain": one workgroup per ciphertext (the reference's launch
-    shape); "batched": two ciphertexts per workgroup sharing every key load.
-    Both give bit-identical results."""
+    shape); "batched": two ciphertexts per workgroup sharing every key load;
+    "auto": the autotuner's choice for this shape (timed on first use; inside
+    a HIP-graph capture an untuned shape takes "batched").  All forms give
+    bit-identical results."""
+    if form == "auto":
+        dims = tuple(int(d) for d in bsk_reshaped.shape[-6:])
+        batch = indices.numel() // (dims[0] * dims[4]) if dims[0] * dims[4] else 0
+        key = (ct_init.shape[-1], l, dims[0], dims[4], _bucket(batch))
+        if key in _TUNED:
+            form = _TUNED[key]
+        elif torch.cuda.is_current_stream_capturing() or batch == 0:
+            form = "batched"
+        else:
+            times = {}
+            out = None
+            for f in FORMS:
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                out = mul_bootstrap(ct_init, bsk_reshaped, all_rot_fft, indices, log_beta, log_q,
+                                    l, f)
+                b.record()
+                b.synchronize()
+                times[f] = a.elapsed_time(b)
+            _TUNED[key] = min(times, key=times.get)
+            return out
     if form not in FORMS:
-        raise _core.TFHEInvalidArgument(f"form must be one of {FORMS}")
+        raise _core.TFHEInvalidArgument(f"form must be one of {FORMS} or 'auto'")
     devs = [ct_init, bsk_reshaped, indices] + ([all_rot_fft] if all_rot_fft is not None else [])
     _core.require_device(*devs)
     if ct_init.dtype != torch.int64 or indices.dtype != torch.int32:

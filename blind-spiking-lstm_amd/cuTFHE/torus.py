@@ -42,7 +42,7 @@ def poly_fft(polys: torch.Tensor) -> torch.Tensor:
 
 
 def linear_extract(ct: torch.Tensor, weights: torch.Tensor, bias: torch.Tensor | None,
-                   log_q: int, support: torch.Tensor | None = None):
+                   log_q: int, support: torch.Tensor | None = None, check_support: bool = True):
     """Homomorphic Linear + coefficient-0 extraction (Spike_LSTM.py:46-61).
 
     ct (batch, 2, N) or (2, N); weights (out, N) int64 weight polynomials;
@@ -60,7 +60,9 @@ def linear_extract(ct: torch.Tensor, weights: torch.Tensor, bias: torch.Tensor |
     if support is None:
         support = torch.nonzero(torch.any(w != 0, dim=0)).reshape(-1).to(torch.int32)
     support = support.to(device=c.device, dtype=torch.int32).contiguous()
-    if support.numel() and (int(support.min()) < 0 or int(support.max()) >= N):
+    # (callers that built `support` from the weights themselves skip this
+    # device->host check, e.g. inside a captured graph)
+    if check_support and support.numel() and (int(support.min()) < 0 or int(support.max()) >= N):
         raise _core.TFHEInvalidArgument("support entries must lie in [0, N)")
     b = None if bias is None else _i64(bias.reshape(-1))
     masks = torch.empty(batch, out_dim, N, dtype=torch.int64, device=c.device)
@@ -164,7 +166,10 @@ def rotate_poly(x: torch.Tensor, exponents, log_q: int) -> torch.Tensor:
     xi = _i64(x)
     N = xi.shape[-1]
     rows = xi.numel() // N
-    e = torch.as_tensor(exponents, dtype=torch.int64).reshape(-1).to(xi.device).contiguous()
+    if isinstance(exponents, torch.Tensor) and exponents.device == xi.device:
+        e = exponents.to(torch.int64).reshape(-1).contiguous()
+    else:
+        e = torch.as_tensor(exponents, dtype=torch.int64).reshape(-1).to(xi.device).contiguous()
     if rows % e.numel():
         raise _core.TFHEInvalidArgument(f"{e.numel()} exponents for {rows} rows")
     out = torch.empty_like(xi)
@@ -233,3 +238,41 @@ def rotate_extract(br_out: torch.Tensor, body: torch.Tensor | None, log_q: int, 
             None if bd is None else bd[s:e].data_ptr(), e - s, N, log_q, n_lut, n_ext,
             masks[s:e].data_ptr(), bodies[s:e].data_ptr()))
     return masks, bodies
+
+
+def prepare_ks_key(ksk_a: torch.Tensor, ksk_b: torch.Tensor) -> torch.Tensor:
+    """Key-switching key (l, n_in, n_out) / (l, n_in[, 1]) int64 -> the int8
+    limb tiles of the MFMA key switch (bslstm_ks_mfma_prepare_key), once per
+    key."""
+    _core.require_device(ksk_a, ksk_b)
+    ka, kb = _i64(ksk_a), _i64(ksk_b)
+    l, n_in, n_out = ka.shape
+    nbytes = int(_lib().bslstm_ks_mfma_key_bytes(l, n_in, n_out))
+    if nbytes < 0:
+        raise _core.TFHEInvalidArgument("key too large for the MFMA key switch")
+    tiles = torch.empty(nbytes, dtype=torch.int8, device=ka.device)
+    _core.check(_lib().bslstm_ks_mfma_prepare_key(_core.stream_of(ka), ka.data_ptr(), kb.data_ptr(),
+                                                  l, n_in, n_out, tiles.data_ptr()))
+    return tiles
+
+
+def key_switch_mfma(key_tiles: torch.Tensor, a: torch.Tensor, b: torch.Tensor, n_out: int,
+                    log_beta: int, l: int, log_q: int):
+    """Exact LWE key switch on the matrix cores (int8 limb GEMMs):
+    a (B, n_in), b (B,) -> (a' (B, n_out), b' (B,)), bit-identical to
+    key_switch."""
+    _core.require_device(key_tiles, a, b)
+    ai = _i64(a)
+    n_in = ai.shape[-1]
+    batch = ai.numel() // n_in
+    bi = _i64(b).reshape(-1)
+    if bi.numel() != batch:
+        raise _core.TFHEInvalidArgument("one body per mask expected")
+    ws_bytes = int(_lib().bslstm_ks_mfma_workspace_bytes(batch, l, n_in))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.int8, device=ai.device)
+    a_out = torch.empty(batch, n_out, dtype=torch.int64, device=ai.device)
+    b_out = torch.empty(batch, dtype=torch.int64, device=ai.device)
+    _core.check(_lib().bslstm_key_switch_lwe_mfma(
+        _core.stream_of(ai), ai.data_ptr(), bi.data_ptr(), key_tiles.data_ptr(), batch, n_in, n_out,
+        log_beta, l, log_q, ws.data_ptr(), ws_bytes, a_out.data_ptr(), b_out.data_ptr()))
+    return a_out, b_out

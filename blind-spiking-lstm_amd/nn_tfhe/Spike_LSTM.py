@@ -80,7 +80,7 @@ class Linear:
         ct = torch.stack([torus._i64(A).reshape(-1, self.degree),
                           torus._i64(B).reshape(-1, self.degree)], dim=1)
         masks, bodies = torus.linear_extract(ct, self.kernel_polynomial, self.bias, self.log_q,
-                                             self.support)
+                                             self.support, check_support=False)
         out = self.kernel_polynomial.shape[0]
         return masks.reshape(*lead, out, self.degree), bodies.reshape(*lead, out, 1)
 
@@ -113,6 +113,8 @@ class CipherSpikeLSTM:
         self.lut_o_fft = heavyBS.lut_fft(np.asarray(lut_o).reshape(n_lut, G, self.degree_lut))
         li = [torus._i64(_dev(np.rint(np.asarray(p, np.float64)), dev)) for p in lut_i]
         self.lut_i = torch.stack([li[0].reshape(G, -1), li[1].reshape(G, -1)], dim=1)  # (G, 2, N)
+        # device-resident exponent of the H rotation (no host copy inside a step)
+        self._shift = torch.full((1,), input_dim, dtype=torch.int64, device=dev)
 
     def _lwe_add(self, x, y):
         return (torus.axpy(x[0], y[0], 1, 1, self.log_q), torus.axpy(x[1], y[1], 1, 1, self.log_q))
@@ -158,7 +160,7 @@ class CipherSpikeLSTM:
         Cp = self.heavyBS.packing((C_t[0], C_t[1].unsqueeze(-1)))
         Hp = self.heavyBS.packing((H_t[0], H_t[1].unsqueeze(-1)))
         C_new = torch.stack([Cp[0], Cp[1]], dim=1)
-        H_new = torus.rotate_poly(torch.stack([Hp[0], Hp[1]], dim=1), self.input_dim, lq)
+        H_new = torus.rotate_poly(torch.stack([Hp[0], Hp[1]], dim=1), self._shift, lq)
         return H_new, C_new
 
     def __call__(self, x, seq_len=None):

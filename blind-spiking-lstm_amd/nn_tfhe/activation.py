@@ -55,6 +55,17 @@ def _to_device(x, device):
     return torch.as_tensor(np.asarray(x), device=device)
 
 
+def _ks_key(key, dict_params_ks: dict, device):
+    """The LWE key-switching key on the device; an int64 key with digits of
+    at most 7 bits also gets its int8 limb tiles for the matrix-core key
+    switch (a third tuple entry that key_switch_LWE picks up)."""
+    ka, kb = (_to_device(k, device) for k in key[:2])
+    if ka.dtype == torch.int64 and log2_int(dict_params_ks["beta_ks"]) <= 7:
+        kb2 = kb.reshape(ka.shape[0], ka.shape[1])
+        return ka, kb, torus.prepare_ks_key(ka, kb2)
+    return ka, kb
+
+
 class HeavysideBoostrapper:
     """nn_tfhe/activation.py:70-162: key switch to the small LWE key, then
     many-LUT programmable bootstrap."""
@@ -65,7 +76,7 @@ class HeavysideBoostrapper:
         self.device = torch.device(device)
         n = dict_params_ks_LWE["degree"]
         self.bootstrapping_key = prepare_bsk(_to_device(bootstrapping_key, self.device), n, collapse)
-        self.key_switching_key_bs = tuple(_to_device(k, self.device) for k in key_switching_key_bs)
+        self.key_switching_key_bs = _ks_key(key_switching_key_bs, dict_params_ks_LWE, self.device)
         # packing key in the transform layout the packing kernel reads, once
         self.ksk_packing = (None if ksk_packing is None else
                             prepare_ksp_fft(ksk_packing if isinstance(ksk_packing, torch.Tensor)
@@ -140,7 +151,7 @@ class ReluBoostrapper:
         self.device = torch.device(device)
         n = dict_params_ks_LWE["degree"]
         self.bootstrapping_key = prepare_bsk(_to_device(bootstrapping_key, self.device), n, collapse)
-        self.key_switching_key_bs = tuple(_to_device(k, self.device) for k in key_switching_key_bs)
+        self.key_switching_key_bs = _ks_key(key_switching_key_bs, dict_params_ks_LWE, self.device)
         self.ksk_packing = (None if ksk_packing is None else
                             prepare_ksp_fft(ksk_packing if isinstance(ksk_packing, torch.Tensor)
                                             else tuple(_to_device(k, self.device)
@@ -164,11 +175,22 @@ class ReluBoostrapper:
                     torch.as_tensor(body, device=self.device)]
 
     def __call__(self, h):
-        c_lwe_ks = key_switch_LWE(self.key_switching_key_bs, h, self.dict_params_ks_LWE)
+        """h = (masks (..., hidden, N), bodies (..., hidden, 1)): unit u goes
+        through LUT u; leading axes (sequences) share the LUTs."""
+        masks, bodies = h
+        lead, N = masks.shape[:-2], masks.shape[-1]
+        hid = masks.shape[-2]
+        reps = masks.numel() // (hid * N)
+        c_lwe_ks = key_switch_LWE(self.key_switching_key_bs,
+                                  (masks.reshape(-1, N), bodies.reshape(-1, 1)),
+                                  self.dict_params_ks_LWE)
+        lut = [x.unsqueeze(0).expand(reps, hid, x.shape[-1]).reshape(reps * hid, -1)
+               for x in self.LUT]
         p = self.dict_params_lut
-        return cuboot_merge(c_lwe_ks, self.LUT, self.bootstrapping_key, p["q"], p["beta_bs"],
-                            p["l_bs"], p["degree"], self.collapse, self.all_rot_possible_fourier,
-                            None)
+        om, ob = cuboot_merge(c_lwe_ks, lut, self.bootstrapping_key, p["q"], p["beta_bs"],
+                              p["l_bs"], p["degree"], self.collapse, self.all_rot_possible_fourier,
+                              None)
+        return om.reshape(*lead, hid, -1), ob.reshape(*lead, hid, 1)
 
     def packing(self, x):
         p = self.dict_params

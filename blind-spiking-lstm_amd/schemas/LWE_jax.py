@@ -115,17 +115,23 @@ def key_switch_LWE(key_switching_key, ciphertext, dict_params: dict):
     beta = dict_params["beta_ks"]
     l = dict_params["l_ks"]
     q = dict_params["q"]
-    ksk_a, ksk_b = key_switching_key
+    ksk_a, ksk_b = key_switching_key[0], key_switching_key[1]
+    tiles = key_switching_key[2] if len(key_switching_key) > 2 else None
     a, b = ciphertext
     _core.require_device(ksk_a, ksk_b, a, b)
     n_in = a.shape[-1]
     n_out = ksk_a.shape[-1]
     if ksk_a.dtype == torch.int64 or a.dtype == torch.int64 or log2_exact(q) > 32:
-        # exact int64 form (any log2 q <= 64)
+        # exact int64 forms (any log2 q <= 64); with the key's int8 limb tiles
+        # (a third entry, torus.prepare_ks_key) on the matrix cores
         from cuTFHE import torus
         lead = a.shape[:-1]
-        a_out, b_out = torus.key_switch(ksk_a, ksk_b, a.reshape(-1, n_in), b.reshape(-1),
-                                        log2_exact(beta), l, log2_exact(q))
+        if tiles is not None:
+            a_out, b_out = torus.key_switch_mfma(tiles, a.reshape(-1, n_in), b.reshape(-1), n_out,
+                                                 log2_exact(beta), l, log2_exact(q))
+        else:
+            a_out, b_out = torus.key_switch(ksk_a, ksk_b, a.reshape(-1, n_in), b.reshape(-1),
+                                            log2_exact(beta), l, log2_exact(q))
         return a_out.reshape(*lead, n_out), b_out.reshape(*lead, 1)
     if tuple(ksk_a.shape) != (l, n_in, n_out):
         raise _core.TFHEInvalidArgument(f"KSK shape {tuple(ksk_a.shape)} != {(l, n_in, n_out)}")

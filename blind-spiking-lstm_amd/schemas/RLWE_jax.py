@@ -186,7 +186,8 @@ def _blind_rotate_int(pk, lut_ct, bsk, q, beta, l, degree, collapse, all_rot, n_
     else:
         idx = rotation_indices(pk, log_q, degree, collapse, n_lut)
     bsk_k = prepare_bsk(bsk, pk.shape[-1], collapse)
-    return mul_bootstrap(lut_ct.to(torch.int64), bsk_k, all_rot, idx, log_beta, log_q, l)
+    return mul_bootstrap(lut_ct.to(torch.int64), bsk_k, all_rot, idx, log_beta, log_q, l,
+                         form="auto")
 
 
 def rotate_extract(br_out: torch.Tensor, b: torch.Tensor, log_q: int, n_lut: int):

@@ -252,6 +252,36 @@ int bslstm_rotate_poly(void *stream, const int64_t *in, int64_t rows, int64_t de
                        const int64_t *exponents, int64_t rows_per_exponent, int64_t log2_q,
                        int64_t *out);
 
+/* ------------------------------------------------------------------------
+ * LWE key switch on the matrix cores (the exact int8 "split" form).
+ * Same mathematics as bslstm_key_switch_lwe_i64 (schemas/LWE_jax.py:97-117)
+ * and bit-identical results: the digit matrix (int8) times the key cut into
+ * eight balanced signed bytes, as int8 MFMA GEMMs with int32 accumulation,
+ * recombined exactly mod 2^64.  Needs log2_beta_ks <= 7 (int8 digits) and
+ * n_in*l_ks < 2^18 (exact int32 sums).
+ * ------------------------------------------------------------------------ */
+
+/* Bytes of the prepared key (-1 on bad arguments). */
+int64_t bslstm_ks_mfma_key_bytes(int64_t l, int64_t n_in, int64_t n_out);
+
+/* Cut ksk_a int64 (l, n_in, n_out) and ksk_b int64 (l, n_in) (as built by
+ * get_keyswitch_sk_LWE, schemas/LWE_jax.py:86-93) into the int8 limb tiles
+ * the GEMM reads; key_tiles holds bslstm_ks_mfma_key_bytes() bytes.  Once
+ * per key. */
+int bslstm_ks_mfma_prepare_key(void *stream, const int64_t *ksk_a, const int64_t *ksk_b,
+                               int64_t l, int64_t n_in, int64_t n_out, int8_t *key_tiles);
+
+/* Workspace of bslstm_key_switch_lwe_mfma in bytes (the int8 digit tiles). */
+int64_t bslstm_ks_mfma_workspace_bytes(int64_t batch, int64_t l, int64_t n_in);
+
+/* a_in int64 (batch, n_in), b_in int64 (batch) -> a_out int64 (batch, n_out),
+ * b_out int64 (batch); key_tiles from bslstm_ks_mfma_prepare_key. */
+int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, const int64_t *b_in,
+                               const int8_t *key_tiles, int64_t batch, int64_t n_in,
+                               int64_t n_out, int64_t log2_beta_ks, int64_t l_ks, int64_t log2_q,
+                               void *workspace, int64_t workspace_bytes,
+                               int64_t *a_out, int64_t *b_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,23 @@ def test_key_switch_i64_bit_exact(cuda_device, T, log_q, lb, l, n_in, n_out, B):
     np.testing.assert_array_equal(host(gb), wb)
 
 
+@pytest.mark.parametrize("log_q,lb,l,n_in,n_out,B", [(64, 1, 20, 2048, 702, 40), (64, 1, 20, 2048, 702, 700),
+                                                     (64, 7, 9, 300, 97, 70), (32, 4, 5, 1024, 630, 33),
+                                                     (64, 2, 3, 37, 5, 1)])
+def test_key_switch_mfma_bit_exact(cuda_device, T, log_q, lb, l, n_in, n_out, B):
+    """The int8 limb MFMA key switch equals the exact oracle (and so the
+    VALU int64 key switch) bit for bit, including ragged tiles."""
+    rng = np.random.default_rng(l * n_in + B)
+    s_old, s_new = rng.integers(0, 2, n_in), rng.integers(0, 2, n_out)
+    ka, kb = L.keyswitch_key(rng, s_new, s_old, log_q, lb, l, 2.0 ** (log_q - 20))
+    a, b = L.lwe_encrypt(rng, rng.integers(-8, 8, B) << (log_q - 5), s_old, log_q, 2.0 ** (log_q - 30))
+    tiles = T.prepare_ks_key(dev(ka), dev(kb))
+    ga, gb = T.key_switch_mfma(tiles, dev(a), dev(b), n_out, lb, l, log_q)
+    wa, wb = L.key_switch_int(ka, kb, a, b, log_q, lb, l)
+    np.testing.assert_array_equal(host(ga), wa)
+    np.testing.assert_array_equal(host(gb), wb)
+
+
 def test_rotation_indices_i64_bit_exact(cuda_device, T):
     rng = np.random.default_rng(3)
     for (N, n, c, n_lut, lq) in [(2048, 702, 3, 4, 64), (256, 48, 3, 4, 64), (1024, 630, 3, 1, 32)]:
@@ -322,3 +339,33 @@ def test_lstm_timesteps_decrypt_to_the_plaintext_rule(cuda_device):
                 checked += 1
         H, C = decH, decC
     assert checked >= total // 4, (checked, total)
+
+
+def test_engine_uneven_lengths_and_graph_replays_match_eager(cuda_device):
+    """Sequences of lengths 3, 1, 2 through the timestep engine: each final
+    state equals the state of that sequence run alone (eager), and replays of
+    the captured step graphs (with eager work in between) reproduce the eager
+    results bit for bit."""
+    from nn_tfhe.engine import SequenceEngine
+    from schemas.RLWE_jax import encrypt
+    m = _Model(seed=1)
+    lstm, sk = _build_device_layer(m)
+    rng = np.random.default_rng(8)
+    seqs = []
+    for T_ in (3, 1, 2):
+        msg = np.zeros((T_, m.N), np.int64)
+        msg[:, :m.input_dim] = rng.integers(0, 2, (T_, m.input_dim)) << 56
+        A, b = encrypt(msg, sk, m.dp, rng)
+        seqs.append((dev(A), dev(b)))
+    H0, C0 = SequenceEngine(lstm).run(seqs)
+    H0, C0 = host(H0), host(C0)
+    for i, (A, b) in enumerate(seqs):                       # each sequence alone
+        (H1, C1), _ = lstm((A, b))
+        np.testing.assert_array_equal(H0[i], np.stack([host(H1[0]), host(H1[1])]))
+        np.testing.assert_array_equal(C0[i], np.stack([host(C1[0]), host(C1[1])]))
+    eng = SequenceEngine(lstm, use_graph=True)
+    for rep in range(3):
+        Hg, Cg = eng.run(seqs)
+        np.testing.assert_array_equal(host(Hg), H0, err_msg=f"graph replay {rep}")
+        np.testing.assert_array_equal(host(Cg), C0, err_msg=f"graph replay {rep}")
+        SequenceEngine(lstm).run(seqs[1:])                  # eager work between replays
