@@ -343,6 +343,88 @@ def run_classifier(args):
     return out
 
 
+# serving: Poisson arrivals of single-sequence requests (lengths U[4, 128])
+# from 64 clients with their own keys; one LSTM layer (input 128, hidden 128)
+SERVE_CFG = dict(clients=64, slots=48, qps=4.0, duration_s=30.0, len_lo=4, len_hi=128,
+                 input_dim=128, hidden=128, n_lut=4, max_active=96, seed=2)
+
+
+def run_serving(args):
+    """Continuous-batching server at a fixed request rate; every client's keys
+    generated on the device up front (client registration), a key pool of
+    `slots` clients resident at once (LRU eviction beyond that)."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, PKG)
+    import parameters_crypto as PC
+    import setup_crypto as SC
+    from nn_tfhe.serving import EncryptedLSTMServer, KeyPool, Request
+    from schemas.RLWE_jax import encrypt
+
+    cfg = SERVE_CFG
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(cfg["seed"])
+    t0 = time.perf_counter()
+    keys, secrets = {}, {}
+    for c in range(cfg["clients"]):
+        keys[c], secrets[c] = SC.sample_client_keys_device(1000 + c, device=dev)
+    torch.cuda.synchronize()
+    keygen_s = time.perf_counter() - t0
+    hid, nl, K = cfg["hidden"], cfg["n_lut"], cfg["input_dim"] + cfg["hidden"]
+    G = hid // nl
+    factor = PC.q / 16 / 16 / PC.beta_x
+    lin = [SC.make_linear(rng.integers(-7, 8, (K, G)), np.zeros(G), factor, PC.dict_params,
+                          device=dev) for _ in range(3)]
+    s4 = np.full(G, 4.0)
+    gam = [rng.uniform(-2, 2, (nl, G)) for _ in range(3)]
+    lut_f, lut_i, lut_o = SC.get_lut(nl, hid, PC.degree_lut, s4, s4, s4, gam[0], gam[1], gam[2],
+                                     np.ones((nl, G)))
+    margs = dict(dict_params=PC.dict_params, dict_params_lut=PC.dict_params_lut,
+                 dict_params_ks_LWE=PC.dict_params_ks_LWE, dict_params_packing=PC.dict_params_packing,
+                 n_lut=nl, input_dim=cfg["input_dim"], hidden=hid, max_len=cfg["len_hi"],
+                 linear_f=lin[0], linear_i=lin[1], linear_o=lin[2], lut_f=lut_f, lut_i=lut_i,
+                 lut_o=lut_o, beta_x=PC.beta_x)
+    pool = KeyPool(cfg["slots"], keys[0], device=dev)
+    srv = EncryptedLSTMServer(margs, pool, lambda c: keys[c], max_active=cfg["max_active"])
+    # warm-up: one short request per batch-size bucket tunes the launch shapes
+    for S in (1, 2, 4, 8, 16, 32, 64, 96):
+        srv.lstm.heavyBS.set_slots(torch.zeros(S, dtype=torch.int32))
+        z = torch.zeros(S, 2, PC.degree, dtype=torch.int64, device=dev)
+        srv.lstm.step(z, z, z)
+    torch.cuda.synchronize()
+    # client side: Poisson arrivals, encrypted spike frames
+    reqs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / cfg["qps"])
+        if t > cfg["duration_s"]:
+            break
+        c = int(rng.integers(0, cfg["clients"]))
+        L_ = int(rng.integers(cfg["len_lo"], cfg["len_hi"] + 1))
+        msg = np.zeros((L_, PC.degree), np.int64)
+        msg[:, :cfg["input_dim"]] = rng.integers(0, 2, (L_, cfg["input_dim"])) << 56
+        A, b = encrypt(msg, secrets[c][0], PC.dict_params, rng)
+        reqs.append(Request(len(reqs), c, torch.stack([torch.as_tensor(A), torch.as_tensor(b)],
+                                                       dim=1).to(dev), t))
+    w0 = time.perf_counter()
+    done = srv.run(reqs)
+    wall = time.perf_counter() - w0
+    lat = np.array([r.done - r.arrival for r in done])
+    per_step = np.array([(r.done - r.arrival) / r.x.shape[0] for r in done])
+    return {"workload": f"continuous-batching server: Poisson {cfg['qps']} req/s for "
+                        f"{cfg['duration_s']} s, {cfg['clients']} clients with own keys, "
+                        f"{cfg['slots']} key slots, lengths U[{cfg['len_lo']},{cfg['len_hi']}], "
+                        f"one LSTM layer input {cfg['input_dim']} hidden {hid}, q=2^64, N=2048",
+            "requests": len(done), "timesteps": int(sum(r.x.shape[0] for r in done)),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+            "p50_latency_per_timestep_ms": round(float(np.percentile(per_step, 50)) * 1e3, 2),
+            "p99_latency_per_timestep_ms": round(float(np.percentile(per_step, 99)) * 1e3, 2),
+            "bootstraps_per_s": round(srv.bootstraps / wall, 1), "wall_s": round(wall, 2),
+            "ticks": srv.ticks, "key_loads": pool.loads,
+            "key_bytes_per_client": pool.slot_bytes, "key_pool_bytes": pool.footprint_bytes(),
+            "client_keygen_s": round(keygen_s, 2)}
+
+
 def cpu_baseline_child(sample: int):
     """Oracle (NumPy FP64 restatement of the reference algorithm) on a
     bounded sample of the same workload, single-threaded."""
@@ -404,6 +486,8 @@ def main():
                     help="sequences per timestep for the LSTM throughput line")
     ap.add_argument("--classifier", type=int, default=1,
                     help="run the 32-sequence encrypted classifier (0: skip)")
+    ap.add_argument("--serving", type=int, default=1,
+                    help="run the continuous-batching serving workload (0: skip)")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
     args = ap.parse_args()
     if args.cpu_baseline_child:
@@ -418,6 +502,8 @@ def main():
         res["lstm"] = run_lstm(args)
     if args.classifier:
         res["classifier"] = run_classifier(args)
+    if args.serving:
+        res["serving"] = run_serving(args)
     if args.copy_mib:
         import torch
         src = torch.ones(args.copy_mib * 2 ** 18, dtype=torch.float32, device="cuda:0")

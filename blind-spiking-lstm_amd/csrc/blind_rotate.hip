@@ -38,6 +38,17 @@
 
 namespace bslstm {
 
+// Per-ciphertext key selection for batches that mix clients: the key of
+// ciphertext c is bsk + slots[c] * stride (complex elements), slots clamped
+// to [0, count).  slots == nullptr: one key for the whole batch.  With two
+// ciphertexts per workgroup both must use the same slot (the slot of the
+// first is used).
+struct KeySlots {
+    const int32_t *slots;
+    int64_t stride;
+    int32_t count;
+};
+
 // BSK values of one bin pair for pattern p of gadget level lev, output rows
 // a0 .. a0+NA-1: [a][bc2][0/1] = F-bins 2r, 2r+1 of BSK[g, a0+a, 2*lev+bc2, p].
 // The block pointer is uniform across the workgroup and the lane offset (2r)
@@ -91,7 +102,7 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
                                                         const int32_t *__restrict__ indices,
                                                         int64_t *__restrict__ out, int64_t batch,
                                                         int n_iter, int m, int log2_beta,
-                                                        int log2_q)
+                                                        int log2_q, KeySlots ks)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T, BC = 2 * L;
     constexpr int PPT = 2 / CT;                  // bin pairs per thread in the MAC
@@ -109,6 +120,11 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
     for (int c = 0; c < CT; c++)
         ct_of[c] = first + c < batch ? first + c : batch - 1;   // tail: recompute the last, store nothing
 
+    if (ks.slots) {
+        int sl = ks.slots[ct_of[0]];
+        sl = sl < 0 ? 0 : (sl >= ks.count ? ks.count - 1 : sl);
+        bsk += (size_t)sl * ks.stride;
+    }
     fill_root_table<N>(ztab);
 
     // accumulator rows in registers: x[c][i] = ct[c*N + t + i*T]
@@ -306,30 +322,30 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
 template <int N, int L, int CT>
 static int launch_br(hipStream_t stream, const int64_t *ct, int64_t ct_stride, const double *bsk,
                      const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m,
-                     int log2_beta, int log2_q)
+                     int log2_beta, int log2_q, KeySlots ks)
 {
     const int64_t blocks = (batch + CT - 1) / CT;
     if (log2_q <= 32)
         hipLaunchKernelGGL((br_kernel<N, L, CT, true>), dim3((unsigned)blocks), dim3(N / 8 * CT), 0,
                            stream, ct, ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out,
-                           batch, n_iter, m, log2_beta, log2_q);
+                           batch, n_iter, m, log2_beta, log2_q, ks);
     else
         hipLaunchKernelGGL((br_kernel<N, L, CT, false>), dim3((unsigned)blocks), dim3(N / 8 * CT), 0,
                            stream, ct, ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out,
-                           batch, n_iter, m, log2_beta, log2_q);
+                           batch, n_iter, m, log2_beta, log2_q, ks);
     return launch_status("blind rotation");
 }
 
 template <int N, int CT>
 static int dispatch_l(int64_t l, hipStream_t s, const int64_t *ct, int64_t st, const double *bsk,
                       const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m, int lb,
-                      int lq)
+                      int lq, KeySlots ks)
 {
     switch (l) {
-    case 1: return launch_br<N, 1, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
-    case 2: return launch_br<N, 2, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
-    case 3: return launch_br<N, 3, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
-    case 4: return launch_br<N, 4, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq);
+    case 1: return launch_br<N, 1, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
+    case 2: return launch_br<N, 2, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
+    case 3: return launch_br<N, 3, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
+    case 4: return launch_br<N, 4, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
     default: return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported l");
     }
 }
@@ -387,7 +403,8 @@ template <int CT>
 static int run(void *stream, const int64_t *ct_init, int64_t ct_elems, const double *bsk,
                const int64_t *bsk_dims, const double *all_rot_fft, int64_t rot_rows,
                int64_t rot_cols, const int32_t *indices, int64_t idx_elems, int64_t *output,
-               int64_t batch, int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree)
+               int64_t batch, int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree,
+               KeySlots ks = {nullptr, 0, 1})
 {
     clear_error();
     int64_t st = 0;
@@ -398,14 +415,14 @@ static int run(void *stream, const int64_t *ct_init, int64_t ct_elems, const dou
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int ni = (int)bsk_dims[0], mm = (int)bsk_dims[4], lb = (int)log2_beta, lq = (int)log2_q;
     switch (degree) {
-    case 128: return dispatch_l<128, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
-    case 256: return dispatch_l<256, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
-    case 512: return dispatch_l<512, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
-    case 1024: return dispatch_l<1024, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
-    case 2048: return dispatch_l<2048, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
+    case 128: return dispatch_l<128, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
+    case 256: return dispatch_l<256, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
+    case 512: return dispatch_l<512, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
+    case 1024: return dispatch_l<1024, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
+    case 2048: return dispatch_l<2048, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
     default:
         // 4096 x 2 ciphertexts would need all 160 KiB of LDS: one per workgroup
-        return dispatch_l<4096, 1>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq);
+        return dispatch_l<4096, 1>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
     }
 }
 
@@ -432,4 +449,24 @@ extern "C" int bslstm_mul_bootstrap_batched(void *stream, const int64_t *ct_init
 {
     return bslstm::run<2>(stream, ct_init, ct_elems, bsk, bsk_dims, all_rot_fft, rot_rows, rot_cols,
                           indices, idx_elems, output, batch, log2_beta, log2_q, l, degree);
+}
+
+extern "C" int bslstm_mul_bootstrap_slots(void *stream, const int64_t *ct_init, int64_t ct_elems,
+                                          const double *bsk_pool, const int64_t bsk_dims[6],
+                                          const int32_t *slots, int64_t slot_count,
+                                          const int32_t *indices, int64_t idx_elems,
+                                          int64_t *output, int64_t batch, int64_t log2_beta,
+                                          int64_t log2_q, int64_t l, int64_t degree, int64_t form)
+{
+    if (!slots || slot_count < 1 || slot_count > 0x7fffffffLL || !bsk_dims)
+        return bslstm::fail(BSLSTM_INVALID_ARGUMENT, "slots and 1 <= slot_count required");
+    int64_t stride = 1;
+    for (int i = 0; i < 6; i++)
+        stride *= bsk_dims[i];
+    const bslstm::KeySlots ks{slots, stride, (int32_t)slot_count};
+    if (form == 1)
+        return bslstm::run<1>(stream, ct_init, ct_elems, bsk_pool, bsk_dims, nullptr, 0, 0, indices,
+                              idx_elems, output, batch, log2_beta, log2_q, l, degree, ks);
+    return bslstm::run<2>(stream, ct_init, ct_elems, bsk_pool, bsk_dims, nullptr, 0, 0, indices,
+                          idx_elems, output, batch, log2_beta, log2_q, l, degree, ks);
 }

@@ -149,13 +149,21 @@ __global__ void ks_out_final_kernel(int batch, int n_out, int lq, int64_t *__res
     }
 }
 
+// tile_slots (optional): the key of row tile mt is ktiles + tile_slots[mt] *
+// slot_bytes (clamped to [0, slot_count)), for batches that mix clients
 __global__ __launch_bounds__(64 * KM_WAVES) void ks_mfma_kernel(
     const int8_t *__restrict__ dtiles, const int8_t *__restrict__ ktiles, int batch, int n_out,
-    int KT, int NT, int kt_per_split, int64_t *__restrict__ a_out, int64_t *__restrict__ b_out)
+    int KT, int NT, int kt_per_split, const int32_t *__restrict__ tile_slots, int64_t slot_bytes,
+    int slot_count, int64_t *__restrict__ a_out, int64_t *__restrict__ b_out)
 {
     const int lane = threadIdx.x & 63;
     const int nt = blockIdx.x * KM_WAVES + (threadIdx.x >> 6);
     const int mt = blockIdx.y;
+    if (tile_slots) {
+        int sl = tile_slots[mt];
+        sl = sl < 0 ? 0 : (sl >= slot_count ? slot_count - 1 : sl);
+        ktiles += (size_t)sl * slot_bytes;
+    }
     const int kt0 = blockIdx.z * kt_per_split;
     const int kt1 = min(KT, kt0 + kt_per_split);
     if (nt >= NT)
@@ -233,11 +241,11 @@ extern "C" int64_t bslstm_ks_mfma_workspace_bytes(int64_t batch, int64_t l, int6
     return ((batch + 31) / 32) * (int64_t)tiles_k(n_in, l) * 1024;
 }
 
-extern "C" int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, const int64_t *b_in,
-                                          const int8_t *key_tiles, int64_t batch, int64_t n_in,
-                                          int64_t n_out, int64_t log2_beta_ks, int64_t l_ks,
-                                          int64_t log2_q, void *workspace, int64_t workspace_bytes,
-                                          int64_t *a_out, int64_t *b_out)
+static int ks_mfma_run(void *stream, const int64_t *a_in, const int64_t *b_in,
+                       const int8_t *key_tiles, const int32_t *tile_slots, int64_t slot_count,
+                       int64_t batch, int64_t n_in, int64_t n_out, int64_t log2_beta_ks,
+                       int64_t l_ks, int64_t log2_q, void *workspace, int64_t workspace_bytes,
+                       int64_t *a_out, int64_t *b_out)
 {
     clear_error();
     if (log2_q < 2 || log2_q > 64)
@@ -270,11 +278,36 @@ extern "C" int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, con
     const int kps = (KT + ks - 1) / ks;
     ks = (KT + kps - 1) / kps;
     hipLaunchKernelGGL(ks_mfma_kernel, dim3(gx, MT, ks), dim3(64 * KM_WAVES), 0, s, dt, key_tiles,
-                       (int)batch, (int)n_out, KT, NT, kps, a_out, b_out);
+                       (int)batch, (int)n_out, KT, NT, kps, tile_slots,
+                       bslstm_ks_mfma_key_bytes(l_ks, n_in, n_out), (int)slot_count, a_out, b_out);
     if (int st = launch_status("key switch (MFMA)"))
         return st;
     if (log2_q < 64)
         hipLaunchKernelGGL(ks_out_final_kernel, dim3(eb), dim3(256), 0, s, (int)batch, (int)n_out,
                            (int)log2_q, a_out, b_out);
     return launch_status("key switch (MFMA)");
+}
+
+extern "C" int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, const int64_t *b_in,
+                                          const int8_t *key_tiles, int64_t batch, int64_t n_in,
+                                          int64_t n_out, int64_t log2_beta_ks, int64_t l_ks,
+                                          int64_t log2_q, void *workspace, int64_t workspace_bytes,
+                                          int64_t *a_out, int64_t *b_out)
+{
+    return ks_mfma_run(stream, a_in, b_in, key_tiles, nullptr, 1, batch, n_in, n_out, log2_beta_ks,
+                       l_ks, log2_q, workspace, workspace_bytes, a_out, b_out);
+}
+
+extern "C" int bslstm_key_switch_lwe_mfma_slots(void *stream, const int64_t *a_in,
+                                                const int64_t *b_in, const int8_t *key_pool,
+                                                const int32_t *tile_slots, int64_t slot_count,
+                                                int64_t batch, int64_t n_in, int64_t n_out,
+                                                int64_t log2_beta_ks, int64_t l_ks, int64_t log2_q,
+                                                void *workspace, int64_t workspace_bytes,
+                                                int64_t *a_out, int64_t *b_out)
+{
+    if (!tile_slots || slot_count < 1 || slot_count > 0x7fffffffLL)
+        return fail(BSLSTM_INVALID_ARGUMENT, "tile_slots and 1 <= slot_count required");
+    return ks_mfma_run(stream, a_in, b_in, key_pool, tile_slots, slot_count, batch, n_in, n_out,
+                       log2_beta_ks, l_ks, log2_q, workspace, workspace_bytes, a_out, b_out);
 }

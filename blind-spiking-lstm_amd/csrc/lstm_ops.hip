@@ -148,12 +148,16 @@ __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
 // (coefficients t + i*N/8), the automorphism permutes it through LDS.
 // gkey is (log2 N, l, 2, N/2) complex in the reference's bin order.
 // ---------------------------------------------------------------------------
+// slots (optional): LWE w uses the Galois key gkey + slots[w] * (log2 N * l *
+// 2 * N/2), clamped to [0, slot_count) (batches that mix clients).
 template <int N>
 __global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__restrict__ lwe_a,
                                                            const int64_t *__restrict__ lwe_b,
                                                            const cplx *__restrict__ gkey, int l,
                                                            int lb, int lq,
-                                                           int64_t *__restrict__ out)
+                                                           int64_t *__restrict__ out,
+                                                           const int32_t *__restrict__ slots,
+                                                           int slot_count)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T, LOGN = __builtin_ctz(N);
     __shared__ cplx buf[2 * M];
@@ -161,6 +165,11 @@ __global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__rest
     int64_t *xb = reinterpret_cast<int64_t *>(buf);   // 2N int64 for the permutation
     const int t = threadIdx.x;
     const size_t wid = blockIdx.x;
+    if (slots) {
+        int sl = slots[wid];
+        sl = sl < 0 ? 0 : (sl >= slot_count ? slot_count - 1 : sl);
+        gkey += (size_t)sl * LOGN * l * 2 * M;
+    }
     fill_root_table<N>(ztab);
 
     uint64_t A[8], B[8];
@@ -427,18 +436,27 @@ __global__ __launch_bounds__(256) void pack_transpose_kernel(const int64_t *__re
     }
 }
 
+// slots (optional): packing p uses the key ksk + slots[p] * (n_in * l * 2 *
+// N/2), clamped to [0, slot_count) (batches that mix clients).
 template <int N>
 __global__ __launch_bounds__(N / 8) void packing_mac_kernel(const int64_t *__restrict__ rows,
                                                             int n_in, int rows_per_block,
                                                             const cplx *__restrict__ ksk, int l,
                                                             int lb, int lq,
-                                                            cplx *__restrict__ partial)
+                                                            cplx *__restrict__ partial,
+                                                            const int32_t *__restrict__ slots,
+                                                            int slot_count)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
     __shared__ cplx buf[2 * M];
     __shared__ cplx ztab[N];
     const int t = threadIdx.x;
     const int p = blockIdx.y, blk = blockIdx.x, NB = gridDim.x;
+    if (slots) {
+        int sl = slots[p];
+        sl = sl < 0 ? 0 : (sl >= slot_count ? slot_count - 1 : sl);
+        ksk += (size_t)sl * n_in * l * 2 * M;
+    }
     fill_root_table<N>(ztab);
     __syncthreads();
     const int j0 = blk * rows_per_block;
@@ -648,10 +666,9 @@ extern "C" int bslstm_linear_extract(void *stream, const int64_t *ct, int64_t ct
     return launch_status("linear extract");
 }
 
-extern "C" int bslstm_pack_lwe_to_rlwe(void *stream, const int64_t *lwe_a, const int64_t *lwe_b,
-                                       int64_t count, const double *galois_key_fft,
-                                       int64_t degree, int64_t log2_beta, int64_t l,
-                                       int64_t log2_q, int64_t *out)
+static int pack_lwe_run(void *stream, const int64_t *lwe_a, const int64_t *lwe_b, int64_t count,
+                        const double *galois_key_fft, const int32_t *slots, int64_t slot_count,
+                        int64_t degree, int64_t log2_beta, int64_t l, int64_t log2_q, int64_t *out)
 {
     clear_error();
     if (!degree_ok(degree))
@@ -668,8 +685,29 @@ extern "C" int bslstm_pack_lwe_to_rlwe(void *stream, const int64_t *lwe_a, const
     BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((trace_pack_kernel<NN>), dim3((unsigned)count),
                                                     dim3(NN / 8), 0, (hipStream_t)stream, lwe_a,
                                                     lwe_b, gk, (int)l, (int)log2_beta,
-                                                    (int)log2_q, out));
+                                                    (int)log2_q, out, slots, (int)slot_count));
     return launch_status("trace packing");
+}
+
+extern "C" int bslstm_pack_lwe_to_rlwe(void *stream, const int64_t *lwe_a, const int64_t *lwe_b,
+                                       int64_t count, const double *galois_key_fft,
+                                       int64_t degree, int64_t log2_beta, int64_t l,
+                                       int64_t log2_q, int64_t *out)
+{
+    return pack_lwe_run(stream, lwe_a, lwe_b, count, galois_key_fft, nullptr, 1, degree, log2_beta,
+                        l, log2_q, out);
+}
+
+extern "C" int bslstm_pack_lwe_to_rlwe_slots(void *stream, const int64_t *lwe_a,
+                                             const int64_t *lwe_b, int64_t count,
+                                             const double *galois_key_pool, const int32_t *slots,
+                                             int64_t slot_count, int64_t degree, int64_t log2_beta,
+                                             int64_t l, int64_t log2_q, int64_t *out)
+{
+    if (!slots || slot_count < 1 || slot_count > 0x7fffffffLL)
+        return fail(BSLSTM_INVALID_ARGUMENT, "slots and 1 <= slot_count required");
+    return pack_lwe_run(stream, lwe_a, lwe_b, count, galois_key_pool, slots, slot_count, degree,
+                        log2_beta, l, log2_q, out);
 }
 
 extern "C" int bslstm_lut_product_sum(void *stream, const int64_t *rlwe, const double *lut_fft,
@@ -727,10 +765,10 @@ extern "C" int64_t bslstm_packing_workspace_bytes(int64_t packs, int64_t n_in, i
     return packs * n_in * degree * 8 + packs * nb * 2 * (degree / 2) * 16;
 }
 
-extern "C" int bslstm_packing(void *stream, const int64_t *masks, const int64_t *bodies,
-                              int64_t packs, int64_t n_lwes, int64_t n_in, const double *ksk_fft,
-                              int64_t degree, int64_t log2_beta, int64_t l, int64_t log2_q,
-                              void *workspace, int64_t workspace_bytes, int64_t *out)
+static int packing_run(void *stream, const int64_t *masks, const int64_t *bodies, int64_t packs,
+                       int64_t n_lwes, int64_t n_in, const double *ksk_fft, const int32_t *slots,
+                       int64_t slot_count, int64_t degree, int64_t log2_beta, int64_t l,
+                       int64_t log2_q, void *workspace, int64_t workspace_bytes, int64_t *out)
 {
     clear_error();
     if (!degree_ok(degree))
@@ -761,13 +799,36 @@ extern "C" int bslstm_packing(void *stream, const int64_t *masks, const int64_t 
     const cplx *kf = reinterpret_cast<const cplx *>(ksk_fft);
     BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((packing_mac_kernel<NN>), dim3(nb, (unsigned)packs),
                                                     dim3(NN / 8), 0, s, rows, (int)n_in, rpb, kf,
-                                                    (int)l, (int)log2_beta, (int)log2_q, partial));
+                                                    (int)l, (int)log2_beta, (int)log2_q, partial,
+                                                    slots, (int)slot_count));
     if (int st = launch_status("packing MAC"))
         return st;
     BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((packing_finalize_kernel<NN>), dim3((unsigned)packs),
                                                     dim3(NN / 8), 0, s, partial, nb, bodies,
                                                     (int)n_lwes, (int)log2_q, out));
     return launch_status("packing finalize");
+}
+
+extern "C" int bslstm_packing(void *stream, const int64_t *masks, const int64_t *bodies,
+                              int64_t packs, int64_t n_lwes, int64_t n_in, const double *ksk_fft,
+                              int64_t degree, int64_t log2_beta, int64_t l, int64_t log2_q,
+                              void *workspace, int64_t workspace_bytes, int64_t *out)
+{
+    return packing_run(stream, masks, bodies, packs, n_lwes, n_in, ksk_fft, nullptr, 1, degree,
+                       log2_beta, l, log2_q, workspace, workspace_bytes, out);
+}
+
+extern "C" int bslstm_packing_slots(void *stream, const int64_t *masks, const int64_t *bodies,
+                                    int64_t packs, int64_t n_lwes, int64_t n_in,
+                                    const double *ksk_pool, const int32_t *slots,
+                                    int64_t slot_count, int64_t degree, int64_t log2_beta,
+                                    int64_t l, int64_t log2_q, void *workspace,
+                                    int64_t workspace_bytes, int64_t *out)
+{
+    if (!slots || slot_count < 1 || slot_count > 0x7fffffffLL)
+        return fail(BSLSTM_INVALID_ARGUMENT, "slots and 1 <= slot_count required");
+    return packing_run(stream, masks, bodies, packs, n_lwes, n_in, ksk_pool, slots, slot_count,
+                       degree, log2_beta, l, log2_q, workspace, workspace_bytes, out);
 }
 
 extern "C" int bslstm_torus_axpy(void *stream, const int64_t *x, const int64_t *y, int64_t n,

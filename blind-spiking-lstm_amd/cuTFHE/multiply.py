@@ -74,7 +74,7 @@ def tuned_forms() -> dict:
 
 def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft: torch.Tensor | None,
                   indices: torch.Tensor, log_beta: int, log_q: int, l: int,
-                  form: str = "batched") -> torch.Tensor:
+                  form: str = "batched", slots: torch.Tensor | None = None) -> torch.Tensor:
     """The raw custom call of cuTFHE/multiply.py:17-51 (handler
     csrc/mul_bootstrap.cuh:192-243): int64 ct (…,2,N) -> int64 (batch,2,N).
 
@@ -82,11 +82,15 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     shape); "batched": two ciphertexts per workgroup sharing every key load;
     "auto": the autotuner's choice for this shape (timed on first use; inside
     a HIP-graph capture an untuned shape takes "batched").  All forms give
-    bit-identical results."""
+    bit-identical results.
+
+    slots (int32, one per ciphertext): bsk_reshaped is then a pool
+    (C, n_iter, 2, l, 2, m, N/2) of client keys and ciphertext i uses
+    pool[slots[i]]; in the batched form consecutive pairs must share a slot."""
     if form == "auto":
         dims = tuple(int(d) for d in bsk_reshaped.shape[-6:])
         batch = indices.numel() // (dims[0] * dims[4]) if dims[0] * dims[4] else 0
-        key = (ct_init.shape[-1], l, dims[0], dims[4], _bucket(batch))
+        key = (ct_init.shape[-1], l, dims[0], dims[4], _bucket(batch), slots is not None)
         if key in _TUNED:
             form = _TUNED[key]
         elif torch.cuda.is_current_stream_capturing() or batch == 0:
@@ -98,7 +102,7 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
                 a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 a.record()
                 out = mul_bootstrap(ct_init, bsk_reshaped, all_rot_fft, indices, log_beta, log_q,
-                                    l, f)
+                                    l, f, slots)
                 b.record()
                 b.synchronize()
                 times[f] = a.elapsed_time(b)
@@ -121,6 +125,16 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     batch = idx.numel() // (n_iter * m) if n_iter * m else 0
     out = torch.empty(batch, 2, degree, dtype=torch.int64, device=ct.device)
     arr = (_core._I64 * 6)(*dims)
+    if slots is not None:
+        _core.require_device(slots)
+        sl = slots.to(torch.int32).reshape(-1).contiguous()
+        if bsk.dim() != 7 or sl.numel() != batch:
+            raise _core.TFHEInvalidArgument("slots need a (C, 6-D) key pool and one slot per ciphertext")
+        _core.check(_core.lib().bslstm_mul_bootstrap_slots(
+            _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr, sl.data_ptr(),
+            bsk.shape[0], idx.data_ptr(), idx.numel(), out.data_ptr(), batch, log_beta, log_q, l,
+            degree, 1 if form == "plain" else 2))
+        return out
     rot = None if all_rot_fft is None else all_rot_fft.contiguous()
     rot_rows, rot_cols = (0, 0) if rot is None else (rot.shape[-2], rot.shape[-1])
     entry = _core.lib().bslstm_mul_bootstrap if form == "plain" else \

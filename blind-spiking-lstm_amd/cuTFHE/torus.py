@@ -276,3 +276,80 @@ def key_switch_mfma(key_tiles: torch.Tensor, a: torch.Tensor, b: torch.Tensor, n
         _core.stream_of(ai), ai.data_ptr(), bi.data_ptr(), key_tiles.data_ptr(), batch, n_in, n_out,
         log_beta, l, log_q, ws.data_ptr(), ws_bytes, a_out.data_ptr(), b_out.data_ptr()))
     return a_out, b_out
+
+
+# ---------------------------------------------------------------------------
+# key-slot forms: one launch over a batch that mixes clients, each item
+# reading its client's key from a pool of equally shaped keys
+# ---------------------------------------------------------------------------
+
+
+def _slots(slots: torch.Tensor, count: int, pool_slots: int) -> torch.Tensor:
+    sl = slots.to(torch.int32).reshape(-1).contiguous()
+    if sl.numel() != count:
+        raise _core.TFHEInvalidArgument(f"{sl.numel()} slots for {count} items")
+    del pool_slots      # kernels clamp slot indices to the pool
+    return sl
+
+
+def pack_lwe_to_rlwe_slots(a, b, gk_pool: torch.Tensor, slots: torch.Tensor, log_beta: int,
+                           l: int, log_q: int) -> torch.Tensor:
+    """pack_lwe_to_rlwe with LWE i using the Galois key gk_pool[slots[i]]
+    (gk_pool (C, log2 N, l, 2, N/2))."""
+    _core.require_device(a, b, gk_pool, slots)
+    ai = _i64(a)
+    N = ai.shape[-1]
+    cnt = ai.numel() // N
+    bi = _i64(b).reshape(-1)
+    gk = gk_pool.contiguous()
+    logn = N.bit_length() - 1
+    if tuple(gk.shape[1:]) != (logn, l, 2, N // 2) or bi.numel() != cnt:
+        raise _core.TFHEInvalidArgument("galois key pool / LWE shapes do not match")
+    sl = _slots(slots, cnt, gk.shape[0])
+    out = torch.empty(*ai.shape[:-1], 2, N, dtype=torch.int64, device=ai.device)
+    _core.check(_lib().bslstm_pack_lwe_to_rlwe_slots(
+        _core.stream_of(ai), ai.data_ptr(), bi.data_ptr(), cnt, gk.data_ptr(), sl.data_ptr(),
+        gk.shape[0], N, log_beta, l, log_q, out.data_ptr()))
+    return out
+
+
+def packing_slots(masks, bodies, ksp_pool: torch.Tensor, slots: torch.Tensor, log_beta: int,
+                  l: int, log_q: int) -> torch.Tensor:
+    """packing with packing p using ksp_pool[slots[p]] (ksp_pool (C, n_in, l, 2, N/2))."""
+    _core.require_device(masks, bodies, ksp_pool, slots)
+    m = _i64(masks)
+    P, n_lwes, n_in = m.shape
+    bd = _i64(bodies).reshape(P, n_lwes)
+    kp = ksp_pool.contiguous()
+    if kp.dim() != 5 or kp.shape[1] != n_in or kp.shape[2] != l:
+        raise _core.TFHEInvalidArgument("packing key pool shape does not match")
+    N = 2 * kp.shape[4]
+    sl = _slots(slots, P, kp.shape[0])
+    ws_bytes = int(_lib().bslstm_packing_workspace_bytes(P, n_in, N))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=m.device)
+    out = torch.empty(P, 2, N, dtype=torch.int64, device=m.device)
+    _core.check(_lib().bslstm_packing_slots(
+        _core.stream_of(m), m.data_ptr(), bd.data_ptr(), P, n_lwes, n_in, kp.data_ptr(),
+        sl.data_ptr(), kp.shape[0], N, log_beta, l, log_q, ws.data_ptr(), ws_bytes, out.data_ptr()))
+    return out
+
+
+def key_switch_mfma_slots(tiles_pool: torch.Tensor, tile_slots: torch.Tensor, a, b, n_out: int,
+                          log_beta: int, l: int, log_q: int):
+    """key_switch_mfma with row tile t (32 rows) using tiles_pool[tile_slots[t]]
+    (tiles_pool (C, key bytes) int8)."""
+    _core.require_device(tiles_pool, tile_slots, a, b)
+    ai = _i64(a)
+    n_in = ai.shape[-1]
+    batch = ai.numel() // n_in
+    bi = _i64(b).reshape(-1)
+    sl = _slots(tile_slots, (batch + 31) // 32, tiles_pool.shape[0])
+    ws_bytes = int(_lib().bslstm_ks_mfma_workspace_bytes(batch, l, n_in))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.int8, device=ai.device)
+    a_out = torch.empty(batch, n_out, dtype=torch.int64, device=ai.device)
+    b_out = torch.empty(batch, dtype=torch.int64, device=ai.device)
+    _core.check(_lib().bslstm_key_switch_lwe_mfma_slots(
+        _core.stream_of(ai), ai.data_ptr(), bi.data_ptr(), tiles_pool.data_ptr(), sl.data_ptr(),
+        tiles_pool.shape[0], batch, n_in, n_out, log_beta, l, log_q, ws.data_ptr(), ws_bytes,
+        a_out.data_ptr(), b_out.data_ptr()))
+    return a_out, b_out

@@ -79,3 +79,121 @@ def get_tfhe_activation(bsk, key_switching_key_bs, ksk_packing, galois_key, s_he
                            dpl["collapse"], P.all_rot_possible_fourier, P.beta_x, s_head1,
                            device=device)
     return heavy, relu
+
+
+# ---------------------------------------------------------------------------
+# device key generation (serving: one key set per client, generated on the
+# GPU in the layouts the kernels read)
+# ---------------------------------------------------------------------------
+
+def _wrap_const(v: int) -> int:
+    """Python int -> the int64 with the same class mod 2^64."""
+    v &= (1 << 64) - 1
+    return v - (1 << 64) if v >= 1 << 63 else v
+
+
+def _uniform64(shape, gen, device):
+    import torch
+    hi = torch.randint(-2 ** 31, 2 ** 31, shape, generator=gen, device=device, dtype=torch.int64)
+    lo = torch.randint(0, 2 ** 32, shape, generator=gen, device=device, dtype=torch.int64)
+    return hi * (1 << 32) + lo
+
+
+def _negacyclic_mul_key(a, s):
+    """Exact a(X)*s(X) mod (X^N+1, 2^64) for int64 a (..., N) and a binary
+    key s (N,) on the device: four 16-bit limbs of a, each product rounded
+    from an FP64 transform (|coefficients| < 2^16 * N, exact)."""
+    import torch
+    N = a.shape[-1]
+    k = torch.arange(N, device=a.device, dtype=torch.float64)
+    tw = torch.polar(torch.ones_like(k), -torch.pi * k / N)
+    fs = torch.fft.fft(s.to(torch.float64) * tw)
+    acc = torch.zeros_like(a)
+    for limb in range(4):
+        part = ((a >> (16 * limb)) & 0xFFFF).to(torch.float64)
+        c = torch.fft.ifft(torch.fft.fft(part * tw) * fs) / tw
+        acc = acc + torch.round(c.real).to(torch.int64) * _wrap_const(1 << (16 * limb))
+    return acc
+
+
+def _encrypt_rlwe_dev(msg, S, sigma, gen):
+    import torch
+    A = _uniform64(tuple(msg.shape), gen, msg.device)
+    e = torch.round(torch.randn(tuple(msg.shape), generator=gen, device=msg.device,
+                                dtype=torch.float64) * sigma).to(torch.int64)
+    return A, _negacyclic_mul_key(A, S) + msg + e
+
+
+def _glev_dev(x, log_beta: int, l: int):
+    """x * 2^(64 - i*log_beta), i = 1..l, as int64 classes mod 2^64: (l, *x.shape)."""
+    import torch
+    return torch.stack([x * _wrap_const(1 << (64 - i * log_beta)) for i in range(1, l + 1)])
+
+
+def sample_client_keys_device(seed: int, params=None, device="cuda"):
+    """One client's keys at q = 2^64, generated on the device (own seeded
+    torch generator).  Returns (keys, secrets): keys = dict with
+      bsk    complex128 (n_iter, 2, l, 2, 2^c, N/2)   collapsed BSK, kernel layout
+      ks     int8 tiles of the MFMA key switch (N -> n)
+      gk     complex128 (log2 N, l, 2, N/2)           Galois keys (trace packing)
+      ksp    complex128 (N, l, 2, N/2)                packing key
+    and secrets = (sk (N,), sk_bsk (n,)) as numpy arrays (client side)."""
+    import numpy as np
+    import torch
+    from cuTFHE import torus
+    from cuTFHE.utils import all_binary_vectors
+    dp, dpl, dpk, dpp = params if params is not None else (
+        P.dict_params, P.dict_params_lut, P.dict_params_ks_LWE, P.dict_params_packing)
+    if dp["q"] != 2.0 ** 64:
+        raise ValueError("device key generation is written for q = 2^64")
+    dev = torch.device(device)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    N, n = dp["degree"], dpk["degree"]
+    c = dpl["collapse"]
+    S = torch.randint(0, 2, (N,), generator=gen, device=dev, dtype=torch.int64)
+    s = torch.randint(0, 2, (n,), generator=gen, device=dev, dtype=torch.int64)
+    # collapsed BSK (RLWE_jax.py:162-197)
+    lb, L = int(np.log2(dpl["beta_bs"]) + 1e-6), dpl["l_bs"]
+    kron = torch.as_tensor(all_binary_vectors(c), device=dev, dtype=torch.int64)   # (m, c)
+    ind = (s.reshape(-1, 1, c) == kron[None]).all(-1).to(torch.int64)            # (n_iter, m)
+    n_iter, m = ind.shape
+    left = _glev_dev(-ind[..., None] * S, lb, L)                                  # (L, n_iter, m, N)
+    right_poly = torch.zeros(n_iter, m, N, dtype=torch.int64, device=dev)
+    right_poly[..., 0] = ind
+    right = _glev_dev(right_poly, lb, L)
+    coef = torch.empty(n_iter, 2, L, 2, m, N, dtype=torch.int64, device=dev)
+    for side, msg in ((0, left), (1, right)):
+        A, b = _encrypt_rlwe_dev(msg, S, dpl["sigma"], gen)
+        coef[:, 0, :, side] = A.permute(1, 0, 2, 3)
+        coef[:, 1, :, side] = b.permute(1, 0, 2, 3)
+    bsk = torus.poly_fft(coef)
+    del coef, left, right, right_poly
+    # LWE key-switching key N -> n (LWE_jax.py:86-93)
+    lbk, lk = int(np.log2(dpk["beta_ks"]) + 1e-6), dpk["l_ks"]
+    msg = _glev_dev(-S, lbk, lk)                                                  # (lk, N)
+    ka = _uniform64((lk, N, n), gen, dev)
+    e = (torch.round(torch.randn((lk, N), generator=gen, device=dev, dtype=torch.float64)) *
+         dpk["sigma"]).to(torch.int64)
+    kb = (ka * s).sum(-1) + msg + e
+    ks = torus.prepare_ks_key(ka, kb)
+    del ka, kb
+    # Galois keys (RLWE_jax.py:217-226)
+    lbp, lp = int(np.log2(dpp["beta_ks"]) + 1e-6), dpp["l_ks"]
+    logn = int(np.log2(N) + 1e-6)
+    S_np = S.cpu().numpy()
+    gks = []
+    for i in range(logn):
+        from schemas.polynomial_jax import apply_automorphism
+        sa = torch.as_tensor(apply_automorphism(S_np, 2 ** (logn - i) + 1), device=dev)
+        A, b = _encrypt_rlwe_dev(_glev_dev(-sa, lbp, lp), S, dpp["sigma"], gen)
+        gks.append(torch.stack([A, b], dim=1))                                      # (lp, 2, N)
+    gk = torus.poly_fft(torch.stack(gks))
+    # packing key (RLWE_jax.py:205-214)
+    lbs, ls = int(np.log2(dp["beta_ks"]) + 1e-6), dp["l_ks"]
+    poly = torch.zeros(N, N, dtype=torch.int64, device=dev)
+    poly[:, 0] = S
+    msg = _glev_dev(poly, lbs, ls).permute(1, 0, 2).contiguous()                   # (N, ls, N)
+    A, b = _encrypt_rlwe_dev(msg, S, dp["sigma"], gen)
+    ksp = torus.poly_fft(torch.stack([A, b], dim=2))
+    return {"bsk": bsk, "ks": ks, "gk": gk, "ksp": ksp}, (S_np, s.cpu().numpy())

@@ -369,3 +369,54 @@ def test_engine_uneven_lengths_and_graph_replays_match_eager(cuda_device):
         np.testing.assert_array_equal(host(Hg), H0, err_msg=f"graph replay {rep}")
         np.testing.assert_array_equal(host(Cg), C0, err_msg=f"graph replay {rep}")
         SequenceEngine(lstm).run(seqs[1:])                  # eager work between replays
+
+
+def test_serving_results_same_alone_and_in_the_mix(cuda_device):
+    """Continuous batching across clients: requests from three clients with
+    their own device-generated keys (key slots), served together with
+    admissions mid-flight and a key pool smaller than the client count (so a
+    slot is evicted and reloaded), give bit-identical final states to each
+    request served alone; every final cell state decrypts under its own
+    client's key to small multiples of beta_x."""
+    import setup_crypto as SC
+    from nn_tfhe.serving import EncryptedLSTMServer, KeyPool, Request
+    from schemas.RLWE_jax import _phase, encrypt
+    m = _Model(seed=2)
+    keys, secrets = {}, {}
+    for c in range(3):
+        keys[c], secrets[c] = SC.sample_client_keys_device(100 + c, m.params, device="cuda:0")
+    sv = np.full(m.G, m.s)
+    lut_f, lut_i, lut_o = SC.get_lut(m.nl, m.hidden, m.N, sv, sv, sv, m.gamma["f"], m.gamma["i"],
+                                     m.gamma["o"], m.eta, m.beta_x)
+    lin = [SC.make_linear(m.W[g], m.b[g] * m.beta_x, m.factor(), m.dp, device="cuda:0") for g in "fio"]
+    args = dict(dict_params=m.dp, dict_params_lut=m.dpl, dict_params_ks_LWE=m.dpk,
+                dict_params_packing=m.dpp, n_lut=m.nl, input_dim=m.input_dim, hidden=m.hidden,
+                max_len=8, linear_f=lin[0], linear_i=lin[1], linear_o=lin[2], lut_f=lut_f,
+                lut_i=lut_i, lut_o=lut_o, beta_x=m.beta_x)
+    rng = np.random.default_rng(4)
+    # (client, length, arrival tick): the last one arrives while others run
+    specs = [(0, 3, 0.0), (1, 2, 0.0), (2, 4, 0.0), (0, 2, 1.0)]
+    xs = []
+    for c, T_, _ in specs:
+        msg = np.zeros((T_, m.N), np.int64)
+        msg[:, :m.input_dim] = rng.integers(0, 2, (T_, m.input_dim)) << 56
+        A, b = encrypt(msg, secrets[c][0], m.dp, rng)
+        xs.append(torch.stack([dev(A), dev(b)], dim=1))
+
+    def serve(items, slots):
+        pool = KeyPool(slots, keys[0], device="cuda:0")
+        srv = EncryptedLSTMServer(args, pool, lambda c: keys[c], max_active=8)
+        alone = len(items) == 1
+        reqs = [Request(i, specs[i][0], xs[i], 0.0 if alone else specs[i][2]) for i in items]
+        srv.run(reqs, clock=lambda: float(srv.ticks))           # time counted in ticks
+        return {r.rid: (host(r.result[0]), host(r.result[1])) for r in srv.finished}, pool
+
+    mixed, pool = serve(range(4), slots=2)
+    assert pool.loads >= 3                      # three clients through two slots
+    for i in range(4):
+        alone, _ = serve([i], slots=1)
+        np.testing.assert_array_equal(mixed[i][0], alone[i][0], err_msg=f"H of request {i}")
+        np.testing.assert_array_equal(mixed[i][1], alone[i][1], err_msg=f"C of request {i}")
+        ph = _phase((mixed[i][1][0], mixed[i][1][1]), secrets[specs[i][0]][0], m.dp)[:m.hidden]
+        frac = np.abs(ph / 2.0 ** 56 - np.rint(ph / 2.0 ** 56))
+        assert frac.max() < 0.05 and np.abs(np.rint(ph / 2.0 ** 56)).max() <= specs[i][1]
