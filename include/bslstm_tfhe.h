@@ -282,6 +282,50 @@ int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, const int64_t 
                                void *workspace, int64_t workspace_bytes,
                                int64_t *a_out, int64_t *b_out);
 
+/* ------------------------------------------------------------------------
+ * Key-slot forms: one launch over a batch that mixes clients.  Each client's
+ * key sits in one slot of a device pool of equally shaped keys; every item
+ * names its slot (int32, clamped to [0, slot_count) on the device).  No
+ * counterpart in the reference, which serves one key set per process: these
+ * are for the continuous-batching server (requests of many clients share
+ * every launch of a timestep).
+ * ------------------------------------------------------------------------ */
+
+/* bslstm_mul_bootstrap(_batched) with bsk_pool (slot_count, n_iter, 2, l, 2,
+ * m, N/2) and slots int32 (batch).  form 1 = one ciphertext per workgroup,
+ * 2 = two (then ciphertexts 2i and 2i+1 must share a slot). */
+int bslstm_mul_bootstrap_slots(void *stream, const int64_t *ct_init, int64_t ct_elems,
+                               const double *bsk_pool, const int64_t bsk_dims[6],
+                               const int32_t *slots, int64_t slot_count,
+                               const int32_t *indices, int64_t idx_elems, int64_t *output,
+                               int64_t batch, int64_t log2_beta, int64_t log2_q, int64_t l,
+                               int64_t degree, int64_t form);
+
+/* bslstm_key_switch_lwe_mfma with key_pool (slot_count x key bytes) and
+ * tile_slots int32 (ceil(batch/32)): rows 32t .. 32t+31 use slot
+ * tile_slots[t]. */
+int bslstm_key_switch_lwe_mfma_slots(void *stream, const int64_t *a_in, const int64_t *b_in,
+                                     const int8_t *key_pool, const int32_t *tile_slots,
+                                     int64_t slot_count, int64_t batch, int64_t n_in,
+                                     int64_t n_out, int64_t log2_beta_ks, int64_t l_ks,
+                                     int64_t log2_q, void *workspace, int64_t workspace_bytes,
+                                     int64_t *a_out, int64_t *b_out);
+
+/* bslstm_pack_lwe_to_rlwe with galois_key_pool (slot_count, log2 N, l, 2,
+ * N/2) and slots int32 (count). */
+int bslstm_pack_lwe_to_rlwe_slots(void *stream, const int64_t *lwe_a, const int64_t *lwe_b,
+                                  int64_t count, const double *galois_key_pool,
+                                  const int32_t *slots, int64_t slot_count, int64_t degree,
+                                  int64_t log2_beta, int64_t l, int64_t log2_q, int64_t *out);
+
+/* bslstm_packing with ksk_pool (slot_count, n_in, l, 2, N/2) and slots int32
+ * (packs). */
+int bslstm_packing_slots(void *stream, const int64_t *masks, const int64_t *bodies,
+                         int64_t packs, int64_t n_lwes, int64_t n_in, const double *ksk_pool,
+                         const int32_t *slots, int64_t slot_count, int64_t degree,
+                         int64_t log2_beta, int64_t l, int64_t log2_q, void *workspace,
+                         int64_t workspace_bytes, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

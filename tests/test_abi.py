@@ -78,3 +78,37 @@ def test_python_surface_raises_invalid_argument():
     from cuTFHE import multiply
     with pytest.raises(_core.TFHEInvalidArgument):
         multiply.rotation_indices(torch.zeros(3, 10, dtype=torch.float64), 32, 2048, 3, 1)
+
+
+def test_every_bound_symbol_is_declared():
+    assert set(_core.SIGNATURES) == set(declared_symbols())
+
+
+def test_lstm_path_entry_points_reject_on_host():
+    """Validation of the LSTM-path entry points runs before any GPU work."""
+    lib = _core.lib()
+    bad = _core.BSLSTM_INVALID_ARGUMENT
+    # Linear: unsupported degree, K > degree
+    assert lib.bslstm_linear_extract(None, 8, 2 * 100, 8, 8, 4, None, 1, 4, 100, 64, 8, 8) == bad
+    assert lib.bslstm_linear_extract(None, 8, 2 * 256, 8, 8, 300, None, 1, 4, 256, 64, 8, 8) == bad
+    # trace packing: l * log2_beta > log2_q
+    assert lib.bslstm_pack_lwe_to_rlwe(None, 8, 8, 3, 8, 2048, 4, 20, 64, 8) == bad
+    assert "l*log2_beta" in lib.bslstm_last_error().decode()
+    # packing: workspace too small, n_lwes > degree
+    assert lib.bslstm_packing(None, 8, 8, 1, 40, 256, 8, 256, 8, 4, 64, 8, 16, 8) == bad
+    assert "workspace" in lib.bslstm_last_error().decode()
+    assert lib.bslstm_packing(None, 8, 8, 1, 300, 256, 8, 256, 8, 4, 64, 8, 1 << 30, 8) == bad
+    # MFMA key switch: digits wider than int8
+    assert lib.bslstm_key_switch_lwe_mfma(None, 8, 8, 8, 4, 2048, 702, 8, 4, 64, 8, 1 << 30, 8,
+                                          16) == bad
+    assert "log2_beta_ks <= 7" in lib.bslstm_last_error().decode()
+    # key-slot forms need slots
+    dims = (_core._I64 * 6)(234, 2, 2, 2, 8, 1024)
+    assert lib.bslstm_mul_bootstrap_slots(None, 8, 4096, 8, dims, None, 4, 8, 4 * 234 * 8, 8, 4,
+                                          13, 64, 2, 2048, 2) == bad
+    assert lib.bslstm_packing_slots(None, 8, 8, 1, 40, 256, 8, None, 2, 256, 8, 4, 64, 8, 1 << 30,
+                                    8) == bad
+    # aliasing the key-switch output onto its input is refused
+    assert lib.bslstm_key_switch_lwe_i64(None, 8, 16, 8, 8, 4, 2048, 702, 1, 20, 64, 8, 16) == bad
+    assert _core.lib().bslstm_packing_workspace_bytes(2, 2048, 2048) == \
+        2 * 2048 * 2048 * 8 + 2 * 128 * 2 * 1024 * 16
