@@ -92,10 +92,17 @@ __device__ __forceinline__ void mac_pattern(cplx (&W)[NA][2][CT][2], const cplx 
     }
 }
 
-// T32: log2_q <= 32, the accumulator is held as its class mod 2^32 in
-// 32-bit registers (digit extraction reads bits below log2_q only, so the
-// class is all it needs); otherwise as 64-bit integers.
-template <int N, int L, int CT, bool T32>
+// ACC: how the accumulator sits in registers.
+//  ACC_32  log2_q <= 32: its class mod 2^32 in 32-bit registers.
+//  ACC_HI  log2_q > 32 with l*log2_beta + 1 <= 32: bits [log2_q-32, log2_q) of
+//          the class in 32-bit registers.  The gadget digits read only the top
+//          l*log2_beta bits plus the one rounding bit below them (every offset
+//          term is a multiple of 2^(log2_q-32)), so those bits are all a step
+//          consumes; the full class is written out at the last step.
+//  ACC_64  otherwise: all 64 bits.
+enum { ACC_64 = 0, ACC_32 = 1, ACC_HI = 2 };
+
+template <int N, int L, int CT, int ACC>
 __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__restrict__ ct_init,
                                                         int64_t ct_stride,
                                                         const cplx *__restrict__ bsk,
@@ -128,14 +135,16 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
     fill_root_table<N>(ztab);
 
     // accumulator rows in registers: x[c][i] = ct[c*N + t + i*T]
+    constexpr bool T32 = ACC != ACC_64;
     using acc_t = typename std::conditional<T32, uint32_t, uint64_t>::type;
+    const int shift = ACC == ACC_HI ? log2_q - 32 : 0;   // register = class >> shift
     acc_t x[2][8];
     const int64_t *src = ct_init + ct_of[grp] * ct_stride;
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
         for (int i = 0; i < 8; i++)
-            x[c][i] = (acc_t)src[c * N + t + i * T];
+            x[c][i] = (acc_t)((uint64_t)src[c * N + t + i * T] >> shift);
     __syncthreads();
 
     // decomposition offset (dec_fft.cuh:7-17)
@@ -147,7 +156,7 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
         if (rs >= 0)
             off += 1ULL << rs;
     }
-    const acc_t offs = (acc_t)off;      // off < 2^log2_q
+    const acc_t offs = (acc_t)(off >> shift);      // off < 2^log2_q, a multiple of 2^shift
     const acc_t dmask = (acc_t)((1ULL << log2_beta) - 1);
     const int half_beta = 1 << (log2_beta - 1);
 
@@ -177,7 +186,7 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
 #pragma unroll 1
         for (int lev = 0; lev < L; lev++) {
             // ---- digits of level lev (dec_fft.cuh:19-29), fold, forward transform
-            const int sh = log2_q - (lev + 1) * log2_beta;
+            const int sh = log2_q - (lev + 1) * log2_beta - shift;
             cplx v[2][4];
 #pragma unroll
             for (int c = 0; c < 2; c++) {
@@ -284,12 +293,19 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
                     // rint(val) - q*round_half_up(rint(val)/q) lies in [-q/2, q/2)
                     // and is the class of the reference's round(val - q*rint(val/q))
                     // (the subtraction is exact for |val| < 2^52)
+                    // (also exact for |val| up to 2^80 at log2_q = 64: the products
+                    // are then integer-valued and lo/q + 0.5 is exact)
                     lo = rint(lo);
                     hi = rint(hi);
                     lo = lo - q * floor(fma(lo, inv_q, 0.5));
                     hi = hi - q * floor(fma(hi, inv_q, 0.5));
-                    x[a][r] = (acc_t)(int)lo;
-                    x[a][r + 4] = (acc_t)(int)hi;
+                    if (ACC == ACC_HI && g == n_iter - 1 && first + grp < batch) {
+                        int64_t *dst = out + (first + grp) * 2 * N + a * N;
+                        dst[t + r * T] = (int64_t)lo;
+                        dst[t + (r + 4) * T] = (int64_t)hi;
+                    }
+                    x[a][r] = (acc_t)((uint64_t)(int64_t)lo >> shift);
+                    x[a][r + 4] = (acc_t)((uint64_t)(int64_t)hi >> shift);
                 } else {
                     lo = lo - q * rint(lo * inv_q);
                     hi = hi - q * rint(hi * inv_q);
@@ -306,7 +322,7 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
         }
     }
 
-    if (first + grp < batch) {
+    if (ACC != ACC_HI && first + grp < batch) {
         int64_t *dst = out + (first + grp) * 2 * N;
         // T32: centred representative in [-2^(log2_q-1), 2^(log2_q-1))
         const int sh_out = 64 - log2_q;
@@ -325,14 +341,19 @@ static int launch_br(hipStream_t stream, const int64_t *ct, int64_t ct_stride, c
                      int log2_beta, int log2_q, KeySlots ks)
 {
     const int64_t blocks = (batch + CT - 1) / CT;
+    const cplx *bk = reinterpret_cast<const cplx *>(bsk);
     if (log2_q <= 32)
-        hipLaunchKernelGGL((br_kernel<N, L, CT, true>), dim3((unsigned)blocks), dim3(N / 8 * CT), 0,
-                           stream, ct, ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out,
-                           batch, n_iter, m, log2_beta, log2_q, ks);
+        hipLaunchKernelGGL((br_kernel<N, L, CT, ACC_32>), dim3((unsigned)blocks), dim3(N / 8 * CT),
+                           0, stream, ct, ct_stride, bk, idx, out, batch, n_iter, m, log2_beta,
+                           log2_q, ks);
+    else if (L * log2_beta + 1 <= 32)
+        hipLaunchKernelGGL((br_kernel<N, L, CT, ACC_HI>), dim3((unsigned)blocks), dim3(N / 8 * CT),
+                           0, stream, ct, ct_stride, bk, idx, out, batch, n_iter, m, log2_beta,
+                           log2_q, ks);
     else
-        hipLaunchKernelGGL((br_kernel<N, L, CT, false>), dim3((unsigned)blocks), dim3(N / 8 * CT), 0,
-                           stream, ct, ct_stride, reinterpret_cast<const cplx *>(bsk), idx, out,
-                           batch, n_iter, m, log2_beta, log2_q, ks);
+        hipLaunchKernelGGL((br_kernel<N, L, CT, ACC_64>), dim3((unsigned)blocks), dim3(N / 8 * CT),
+                           0, stream, ct, ct_stride, bk, idx, out, batch, n_iter, m, log2_beta,
+                           log2_q, ks);
     return launch_status("blind rotation");
 }
 
