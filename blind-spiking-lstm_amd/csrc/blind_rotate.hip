@@ -70,7 +70,7 @@ __device__ __forceinline__ void load_pair(cplx (&dst)[NA][2][2], const cplx *__r
 // W[a][bc2][c][.] += BSK(p) * rotation root of ciphertext c, for bins r and
 // M-1-r (the latter in conjugate form, see tfhe_device.h).  Each root serves
 // the NA output rows held.
-template <int N, int CT, int NA>
+template <int N, int CT, int NA, bool FULL>
 __device__ __forceinline__ void mac_pattern(cplx (&W)[NA][2][CT][2], const cplx (&bk)[NA][2][2],
                                             const int32_t *const (&idp)[CT], int p, int r,
                                             const cplx *ztab)
@@ -79,9 +79,9 @@ __device__ __forceinline__ void mac_pattern(cplx (&W)[NA][2][CT][2], const cplx 
     for (int c = 0; c < CT; c++) {
         const int e = idp[c][p];
         const int e0 = e * (4 * r + 1);
-        const cplx rot0 = zeta_pow<N>(ztab, e0);
+        const cplx rot0 = zeta_pow<N, FULL>(ztab, e0);
         // zeta^(e(4(M-1-r)+1)) = zeta^(-e(4r+3)) = conj(zeta^(e(4r+1) + 2e))
-        const cplx rot1 = cconj(zeta_pow<N>(ztab, e0 + 2 * e));
+        const cplx rot1 = cconj(zeta_pow<N, FULL>(ztab, e0 + 2 * e));
 #pragma unroll
         for (int a = 0; a < NA; a++)
 #pragma unroll
@@ -115,8 +115,11 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
     constexpr int PPT = 2 / CT;                  // bin pairs per thread in the MAC
     constexpr int NA = (CT == 1 && N <= 2048) ? 2 : 1;   // output rows sharing one root lookup
     static_assert(CT == 1 || CT == 2, "one or two ciphertexts per workgroup");
+    // two ciphertexts per workgroup: full-period root table (no sign flips;
+    // 128 KiB of LDS in all at N = 2048, still one workgroup per CU)
+    constexpr bool FULL = CT == 2 && N <= 2048;
     __shared__ cplx buf[CT * 2 * M];             // per ciphertext: two polynomials
-    __shared__ cplx ztab[N];
+    __shared__ cplx ztab[RootTab<N, FULL>::SIZE];
 
     const int grp = threadIdx.x / T;             // ciphertext slot of this thread (transforms)
     const int t = threadIdx.x - grp * T;         // thread index within the slot
@@ -132,7 +135,7 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
         sl = sl < 0 ? 0 : (sl >= ks.count ? ks.count - 1 : sl);
         bsk += (size_t)sl * ks.stride;
     }
-    fill_root_table<N>(ztab);
+    fill_root_table<N, FULL>(ztab);
 
     // accumulator rows in registers: x[c][i] = ct[c*N + t + i*T]
     constexpr bool T32 = ACC != ACC_64;
@@ -194,13 +197,13 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
                 for (int r = 0; r < 4; r++) {
                     const int lo = (int)(((acc_t)(x[c][r] + offs) >> sh) & dmask) - half_beta;
                     const int hi = (int)(((acc_t)(x[c][r + 4] + offs) >> sh) & dmask) - half_beta;
-                    v[c][r] = cmul({(double)lo, -(double)hi}, zeta_pow<N>(ztab, t + r * T));   // (lo - i*hi) * zeta^k
+                    v[c][r] = cmul({(double)lo, -(double)hi}, zeta_pow<N, FULL>(ztab, t + r * T));   // (lo - i*hi) * zeta^k
                 }
             }
             if constexpr (Wave<N>::OK)
-                forward_wave<N>(v, mybuf, ztab, t);
+                forward_wave<N, FULL>(v, mybuf, ztab, t);
             else
-                forward_pair<N>(v, mybuf, ztab, t);
+                forward_pair<N, FULL>(v, mybuf, ztab, t);
 
             // ---- external product, level lev (mul_bootstrap.cuh:42-67)
 #pragma unroll 1
@@ -222,10 +225,10 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
 #pragma unroll 1
                     for (int p = 0; p < m; p += 2) {
                         load_pair<L, NA>(bkB, bsk_g, 2 * r, a0, lev, p + 1 < m ? p + 1 : p, m, M);
-                        mac_pattern<N, CT, NA>(W, bkA, idp, p, r, ztab);
+                        mac_pattern<N, CT, NA, FULL>(W, bkA, idp, p, r, ztab);
                         if (p + 1 < m) {
                             load_pair<L, NA>(bkA, bsk_g, 2 * r, a0, lev, p + 2 < m ? p + 2 : p + 1, m, M);
-                            mac_pattern<N, CT, NA>(W, bkB, idp, p + 1, r, ztab);
+                            mac_pattern<N, CT, NA, FULL>(W, bkB, idp, p + 1, r, ztab);
                         }
                     }
                     cplx acc[NA][CT][2];
@@ -279,14 +282,14 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
         __syncthreads();
         cplx w[2][4];
         if constexpr (Wave<N>::OK)
-            inverse_wave<N>(w, mybuf, ztab, t);
+            inverse_wave<N, FULL>(w, mybuf, ztab, t);
         else
-            inverse_pair<N>(w, mybuf, ztab, t);
+            inverse_pair<N, FULL>(w, mybuf, ztab, t);
 #pragma unroll
         for (int a = 0; a < 2; a++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const cplx z = cmul(w[a][r], zeta_pow<N>(ztab, -(t + r * T)));
+                const cplx z = cmul(w[a][r], zeta_pow<N, FULL>(ztab, -(t + r * T)));
                 double lo = z.re * inv_m;
                 double hi = -z.im * inv_m;
                 if constexpr (T32) {

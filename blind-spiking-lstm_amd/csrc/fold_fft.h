@@ -38,14 +38,14 @@ __device__ __forceinline__ void dft4(cplx x[4], bool inv)
 }
 
 // Twiddle then radix-4 butterfly for Stockham span Ns (inputs j + r*M/4).
-template <int N>
+template <int N, bool FULL = false>
 __device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, const cplx *ztab)
 {
     if (Ns > 1) {
         const int base = (Fold<N>::M / Ns) * k;     // zeta exponent of w^k, w = exp(-2 pi i/(4 Ns))
 #pragma unroll
         for (int r = 1; r < 4; r++) {
-            cplx w = zeta_pow<N>(ztab, inv ? -r * base : r * base);
+            cplx w = zeta_pow<N, FULL>(ztab, inv ? -r * base : r * base);
             x[r] = cmul(x[r], w);
         }
     }
@@ -56,7 +56,7 @@ __device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, c
 // t + r*T, t = thread index within the ciphertext's group); result left in
 // buf[p*M + bin] in natural bin order.  All threads of the workgroup call it
 // (it contains workgroup barriers).
-template <int N>
+template <int N, bool FULL = false>
 __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
@@ -104,7 +104,7 @@ __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const c
         const int k = t & (Ns - 1);
 #pragma unroll
         for (int p = 0; p < 2; p++)
-            r4_twiddle<N>(x[p], k, Ns, false, ztab);
+            r4_twiddle<N, FULL>(x[p], k, Ns, false, ztab);
         __syncthreads();
         const int idxD = (t / Ns) * 4 * Ns + k;
 #pragma unroll
@@ -118,7 +118,7 @@ __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const c
 
 // Inverse (unnormalised) transform of buf[p*M + bin]; the result lands in
 // registers, v[p][r] = point t + r*T.  Called by all threads of the workgroup.
-template <int N>
+template <int N, bool FULL = false>
 __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
@@ -133,7 +133,7 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const c
         const int k = t & (Ns - 1);
 #pragma unroll
         for (int p = 0; p < 2; p++)
-            r4_twiddle<N>(x[p], k, Ns, true, ztab);
+            r4_twiddle<N, FULL>(x[p], k, Ns, true, ztab);
         if (!Fold<N>::ODD && Ns * 4 == M) {
             // last pass: idxD == t, outputs at t + r*T
 #pragma unroll
@@ -161,7 +161,7 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const c
                 const int j = t + h * T;
                 cplx x0 = buf[p * M + j];
                 cplx x1 = buf[p * M + j + M / 2];
-                x1 = cmul(x1, zeta_pow<N>(ztab, -4 * j));   // conj(w^j), w = exp(-2 pi i/M)
+                x1 = cmul(x1, zeta_pow<N, FULL>(ztab, -4 * j));   // conj(w^j), w = exp(-2 pi i/M)
                 v[p][h] = cadd(x0, x1);                    // point j
                 v[p][h + 2] = csub(x0, x1);                // point j + M/2
             }
@@ -208,7 +208,7 @@ __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 // (lane l holds points l + 64r).  If FROM_REGS the first pass takes v[p][r]
 // instead of reading LDS.  If TO_REGS (inverse only) the last pass leaves
 // its outputs (points l + 64r) in v instead of LDS.
-template <int N, bool INV, bool FROM_REGS, bool TO_REGS>
+template <int N, bool INV, bool FROM_REGS, bool TO_REGS, bool FULL = false>
 __device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *poly0, cplx *poly1, int rg, int l,
                                             const cplx *ztab)
 {
@@ -233,7 +233,7 @@ __device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *poly0, cplx *
         const int k = l & (Ns - 1);
 #pragma unroll
         for (int p = 0; p < 2; p++)
-            r4_twiddle<N>(x[p], k, Ns, INV, ztab);
+            r4_twiddle<N, FULL>(x[p], k, Ns, INV, ztab);
         if (TO_REGS && Ns == 64) {
 #pragma unroll
             for (int p = 0; p < 2; p++)
@@ -255,14 +255,14 @@ __device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *poly0, cplx *
 
 // Forward transform of two polynomials (v[p][r] = point t + r*T) into
 // buf[p*M + phys(bin)].  Called by all threads of the workgroup.
-template <int N>
+template <int N, bool FULL = false>
 __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = N / 2, W = Wave<N>::W;
     const int wv = t >> 6, l = t & 63;
     if constexpr (W == 1) {
         __syncthreads();
-        wave_passes<N, false, true, false>(v, buf, buf + M, 0, l, ztab);
+        wave_passes<N, false, true, false, FULL>(v, buf, buf + M, 0, l, ztab);
     } else {
         // cross-wave DIF stage: z_k(n) = (sum_r x_{n + r M/W} w_W^{rk}) * w_M^{nk}
         cplx z[2][4];
@@ -274,7 +274,7 @@ __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const c
                 dft4(y, false);                                 // y_k, n = t
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    z[p][k] = k ? cmul(y[k], zeta_pow<N>(ztab, 4 * t * k)) : y[k];
+                    z[p][k] = k ? cmul(y[k], zeta_pow<N, FULL>(ztab, 4 * t * k)) : y[k];
                     pos[k] = region_slot<N>(k, t);
                 }
             } else {   // W == 2: points t, t+128 | t+256, t+384
@@ -282,7 +282,7 @@ __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const c
                 for (int h = 0; h < 2; h++) {
                     const int n = t + 128 * h;
                     z[p][2 * h] = cadd(v[p][h], v[p][h + 2]);                          // k = 0
-                    z[p][2 * h + 1] = cmul(csub(v[p][h], v[p][h + 2]), zeta_pow<N>(ztab, 4 * n));
+                    z[p][2 * h + 1] = cmul(csub(v[p][h], v[p][h + 2]), zeta_pow<N, FULL>(ztab, 4 * n));
                     pos[2 * h] = region_slot<N>(0, n);
                     pos[2 * h + 1] = region_slot<N>(1, n);
                 }
@@ -295,22 +295,22 @@ __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const c
             for (int k = 0; k < 4; k++)
                 buf[p * M + pos[k]] = z[p][k];
         __syncthreads();
-        wave_passes<N, false, false, false>(v, buf, buf + M, wv, l, ztab);
+        wave_passes<N, false, false, false, FULL>(v, buf, buf + M, wv, l, ztab);
     }
     __syncthreads();
 }
 
 // Inverse (unnormalised) transform of buf[p*M + phys(bin)]; the result lands
 // in registers, v[p][r] = point t + r*T.  Called by all threads.
-template <int N>
+template <int N, bool FULL = false>
 __device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = N / 2, W = Wave<N>::W;
     const int wv = t >> 6, l = t & 63;
     if constexpr (W == 1) {
-        wave_passes<N, true, false, true>(v, buf, buf + M, 0, l, ztab);
+        wave_passes<N, true, false, true, FULL>(v, buf, buf + M, 0, l, ztab);
     } else {
-        wave_passes<N, true, false, false>(v, buf, buf + M, wv, l, ztab);
+        wave_passes<N, true, false, false, FULL>(v, buf, buf + M, wv, l, ztab);
         __syncthreads();
         // cross-wave DIT stage: x_{n + r M/W} = sum_k (u_k(n) w_M^{-nk}) w_W^{-rk}
 #pragma unroll
@@ -320,7 +320,7 @@ __device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const c
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const cplx u = buf[p * M + region_slot<N>(k, t)];
-                    y[k] = k ? cmul(u, zeta_pow<N>(ztab, -4 * t * k)) : u;
+                    y[k] = k ? cmul(u, zeta_pow<N, FULL>(ztab, -4 * t * k)) : u;
                 }
                 dft4(y, true);
 #pragma unroll
@@ -331,7 +331,7 @@ __device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const c
                 for (int h = 0; h < 2; h++) {
                     const int n = t + 128 * h;
                     const cplx u0 = buf[p * M + region_slot<N>(0, n)];
-                    const cplx u1 = cmul(buf[p * M + region_slot<N>(1, n)], zeta_pow<N>(ztab, -4 * n));
+                    const cplx u1 = cmul(buf[p * M + region_slot<N>(1, n)], zeta_pow<N, FULL>(ztab, -4 * n));
                     v[p][h] = cadd(u0, u1);
                     v[p][h + 2] = csub(u0, u1);
                 }

@@ -68,26 +68,38 @@ __device__ __forceinline__ int root_slot(int r)
     return r ^ f;
 }
 
-template <int N>
+// FULL: the table holds the whole period, zeta^r for r in [0, 2N) (2N
+// entries, twice the LDS), so a lookup needs no sign flip.
+template <int N, bool FULL = false>
+struct RootTab {
+    static constexpr int SIZE = FULL ? 2 * N : N;
+};
+
+template <int N, bool FULL = false>
 __device__ __forceinline__ void fill_root_table(cplx *ztab)
 {
-    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    for (int r = threadIdx.x; r < RootTab<N, FULL>::SIZE; r += blockDim.x) {
         double s, c;
         sincospi((double)r / (double)N, &s, &c);
         ztab[root_slot<N>(r)] = {c, -s};
     }
 }
 
-// zeta^e for any integer e: zeta^N = -1, so zeta^e = +-zeta^(e mod N), the
-// sign given by bit log2(N) of e.  One 16-byte LDS read and a sign flip of
-// both halves (xor on the high words).
-template <int N>
+// zeta^e for any integer e.  Half-period table: zeta^N = -1, so zeta^e =
+// +-zeta^(e mod N), the sign given by bit log2(N) of e -- one 16-byte LDS
+// read and a sign flip of both halves (xor on the high words).  Full-period
+// table: one read of zeta^(e mod 2N).
+template <int N, bool FULL = false>
 __device__ __forceinline__ cplx zeta_pow(const cplx *ztab, int e)
 {
-    const cplx z = ztab[root_slot<N>(e & (N - 1))];
-    const uint32_t flip = ((uint32_t)e & N) ? 0x80000000u : 0u;
-    return {__hiloint2double((int)(__double2hiint(z.re) ^ flip), __double2loint(z.re)),
-            __hiloint2double((int)(__double2hiint(z.im) ^ flip), __double2loint(z.im))};
+    if constexpr (FULL) {
+        return ztab[root_slot<N>(e & (2 * N - 1))];
+    } else {
+        const cplx z = ztab[root_slot<N>(e & (N - 1))];
+        const uint32_t flip = ((uint32_t)e & N) ? 0x80000000u : 0u;
+        return {__hiloint2double((int)(__double2hiint(z.re) ^ flip), __double2loint(z.re)),
+                __hiloint2double((int)(__double2hiint(z.im) ^ flip), __double2loint(z.im))};
+    }
 }
 
 }  // namespace bslstm
