@@ -255,6 +255,8 @@ def run_lstm(args):
                     "p99_ms": round(float(np.percentile(ms, 99)), 3),
                     "mean_ms": round(float(ms.mean()), 3),
                     "bootstraps_per_s": round(3 * G * S / (ms.mean() / 1e3), 1)}
+    from cuTFHE.multiply import tuned_forms
+    out["tuned_forms"] = {str(k): v for k, v in tuned_forms().items()}
     return out
 
 
@@ -451,6 +453,92 @@ def cpu_baseline_child(sample: int):
     print(json.dumps({"value": sample / dt, "seconds": dt}))
 
 
+def precision_check(sample: int):
+    """Phase error of the GPU blind rotation against the exact int128 oracle
+    and against the NumPy FP64 restatement of the reference algorithm, on
+    `sample` ciphertexts of the batched-PBS workload with real seeded keys.
+    The oracle is the checker here; the GPU kernels are what is measured."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, PKG)
+    from cuTFHE.multiply import mul_bootstrap, rotation_indices
+    from oracle import exact as X
+    from oracle import tfhe_oracle as O
+    from schemas.RLWE_jax import rotate_extract
+
+    cfg = CFG
+    N, n, L, lb, c = cfg["N"], cfg["n"], cfg["L"], cfg["log_beta"], cfg["collapse"]
+    nl, mb, lq = cfg["n_lut"], cfg["msg_bits"], cfg["log_q"]
+    n_iter = n // c
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(5)
+    s = rng.integers(0, 2, n)
+    S = rng.integers(0, 2, N)
+    bsk = O.bsk_collapse_coef(rng, s, S, lq, lb, L, c, cfg["sigma_bsk"])
+    fft_k = np.ascontiguousarray(O.reshape_bsk_for_kernel(O.bsk_to_fourier_ordered(bsk), n_iter, c))
+    coef_k = np.ascontiguousarray(O.bsk_coef_kernel_layout(bsk, n_iter, c))
+    del bsk
+    delta = 1 << (lq - mb - 1)
+    msg = rng.integers(-(1 << (mb - 1)), 1 << (mb - 1), sample)
+    T = O.lut_from_function(N, nl, MANY_LUT_FNS[:nl], mb, delta)
+    a, b = O.lwe_encrypt(rng, msg * delta, s, lq, cfg["sigma_lwe"])
+    lut = np.stack([np.zeros(N, np.int64), np.asarray(T, np.int64)])
+    lut_b = np.ascontiguousarray(np.broadcast_to(lut, (sample, 2, N)))
+
+    dev = torch.device("cuda:0")
+    pk = torch.as_tensor(a.astype(np.float64)).to(dev)
+    bd = torch.as_tensor(b.astype(np.float64)).reshape(sample, 1).to(dev)
+    bsk_t = torch.as_tensor(fft_k).to(dev)
+    lut_t = torch.as_tensor(lut_b).to(dev)
+    idx_t = rotation_indices(pk, lq, N, c, nl)
+    gpu = {}
+    for form in ("batched", "plain"):
+        br = mul_bootstrap(lut_t, bsk_t, None, idx_t, lb, lq, L, form=form)
+        masks, bodies = rotate_extract(br, bd, lq, nl)
+        torch.cuda.synchronize()
+        gpu[form] = (br.cpu().numpy(), masks.cpu().numpy().astype(np.int64),
+                     bodies.cpu().numpy().astype(np.int64).reshape(sample, nl))
+    del bsk_t
+    idx = O.rotation_indices(a, lq, N, c, nl)
+    br_x = X.blind_rotate(lut, coef_k, idx, lb, lq, L)
+    br_f = np.asarray(O.blind_rotate_f64(lut_b, fft_k, idx, lb, lq, L), np.int64)
+
+    def extract(br):
+        m_, b_ = O.cuboot_merge_tail(br, b, lq, nl)
+        return np.asarray(m_, np.int64), np.asarray(b_, np.int64).reshape(sample, nl)
+
+    expect = np.array([[f(int(x)) for f in MANY_LUT_FNS[:nl]] for x in msg], np.int64)
+    mod = 1 << (mb + 1)
+
+    def decrypt(masks, bodies):
+        ph = O.centered_mod(bodies - masks @ S.astype(np.int64), lq)
+        return np.mod(np.rint(ph / delta).astype(np.int64), mod), O.centered_mod(ph - expect * delta, lq)
+
+    dec_g, err_g = decrypt(*gpu["batched"][1:])
+    dec_x, err_x = decrypt(*extract(br_x))
+    dec_f, err_f = decrypt(*extract(br_f))
+    dev_g = O.centered_mod(gpu["batched"][0] - br_x, lq)
+    dev_f = O.centered_mod(br_f - br_x, lq)
+    pred = O.predicted_bs_std(N, n, lb, L, c, cfg["sigma_bsk"], lq)
+    return {
+        "sample": f"{sample} ciphertexts of the batched-PBS workload, seeded keys "
+                  f"(sigma_bsk 2^{math.log2(cfg['sigma_bsk']):.0f}, sigma_lwe "
+                  f"2^{math.log2(cfg['sigma_lwe']):.0f})",
+        "coef_dev_vs_exact": {"gpu_max_abs": int(np.abs(dev_g).max()), "gpu_std": float(dev_g.std()),
+                              "restatement_max_abs": int(np.abs(dev_f).max()),
+                              "restatement_std": float(dev_f.std())},
+        "batched_equals_plain": bool(np.array_equal(gpu["batched"][0], gpu["plain"][0])),
+        "phase_err_std": {"gpu": float(err_g.std()), "exact": float(err_x.std()),
+                          "restatement": float(err_f.std()), "predicted_bs": pred},
+        "phase_err_std_ratio_gpu_vs_restatement": float(err_g.std() / err_f.std()),
+        "messages_identical_gpu_vs_exact": float((dec_g == dec_x).mean()),
+        "messages_identical_gpu_vs_restatement": float((dec_g == dec_f).mean()),
+        "messages_equal_lut": float((dec_g == np.mod(expect, mod)).mean()),
+        "seconds": round(time.perf_counter() - t0, 1),
+    }
+
+
 def cpu_baseline(sample: int):
     env = dict(os.environ)
     for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
@@ -480,6 +568,9 @@ def main():
                     help="also run one elementwise negation over this many MiB (reads and "
                          "writes that many bytes): the calibration kernel for HBM counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision-sample", type=int, default=16,
+                    help="ciphertexts compared against the exact int128 oracle and the FP64 "
+                         "restatement (phase-error statistics; 0: skip)")
     ap.add_argument("--lstm-steps", type=int, default=32,
                     help="timesteps of the encrypted spiking-LSTM layer to time (0: skip)")
     ap.add_argument("--lstm-batch", type=int, default=32,
@@ -512,6 +603,8 @@ def main():
         torch.cuda.synchronize()
         res["copy_calibration_bytes"] = args.copy_mib * 2 ** 20
     res["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
+    if not args.no_cpu_baseline and args.precision_sample > 0:
+        res["precision"] = precision_check(args.precision_sample)
     print(json.dumps(res), flush=True)
 
 

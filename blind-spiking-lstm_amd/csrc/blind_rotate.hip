@@ -26,8 +26,12 @@
 //  * CT ciphertexts per workgroup.  CT = 1 is the plain single-launch form
 //    (one workgroup per ciphertext, as the reference).  CT = 2 is the batched
 //    form: a thread multiplies each BSK value it loads into both
-//    ciphertexts, halving the key traffic from L2.  Both run identical
-//    arithmetic per ciphertext, so their outputs are bit-identical.
+//    ciphertexts, halving the key traffic from L2.  RG = 2 (with CT = 1) is
+//    the latency form: the ciphertext's two accumulator rows sit in two
+//    thread groups, each transforming its own row's digit polynomials, and
+//    the multiply-accumulate spreads over both groups, so a step runs on
+//    twice the threads.  All forms run identical arithmetic per ciphertext,
+//    so their outputs are bit-identical.
 //  * Roots of unity come from a half-period table in LDS (N complex),
 //    filled once per workgroup with the device sincospi: no 2N x N/2
 //    rotation table is read.
@@ -77,11 +81,13 @@ __device__ __forceinline__ void mac_pattern(cplx (&W)[NA][2][CT][2], const cplx 
 {
 #pragma unroll
     for (int c = 0; c < CT; c++) {
-        const int e = idp[c][p];
-        const int e0 = e * (4 * r + 1);
+        // exponents matter mod 2N: a 24-bit multiply (operands < 2^13)
+        const int e = (int)((uint32_t)idp[c][p] & (2 * N - 1));
+        const int e0 = (int)__umul24((uint32_t)e, (uint32_t)(4 * r + 1));
         const cplx rot0 = zeta_pow<N, FULL>(ztab, e0);
-        // zeta^(e(4(M-1-r)+1)) = zeta^(-e(4r+3)) = conj(zeta^(e(4r+1) + 2e))
-        const cplx rot1 = cconj(zeta_pow<N, FULL>(ztab, e0 + 2 * e));
+        // zeta^(e(4(M-1-r)+1)) = zeta^(-e(4r+3)) = conj(zeta^(e(4r+1)) * zeta^(2e)),
+        // zeta^(2e) being uniform across the workgroup (one broadcast read)
+        const cplx rot1 = cconj(cmul(rot0, zeta_pow<N, FULL>(ztab, 2 * e)));
 #pragma unroll
         for (int a = 0; a < NA; a++)
 #pragma unroll
@@ -102,27 +108,34 @@ __device__ __forceinline__ void mac_pattern(cplx (&W)[NA][2][CT][2], const cplx 
 //  ACC_64  otherwise: all 64 bits.
 enum { ACC_64 = 0, ACC_32 = 1, ACC_HI = 2 };
 
-template <int N, int L, int CT, int ACC>
-__global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__restrict__ ct_init,
+template <int N, int L, int CT, int RG, int ACC, int MP>
+__global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *__restrict__ ct_init,
                                                         int64_t ct_stride,
                                                         const cplx *__restrict__ bsk,
                                                         const int32_t *__restrict__ indices,
                                                         int64_t *__restrict__ out, int64_t batch,
-                                                        int n_iter, int m, int log2_beta,
+                                                        int n_iter, int m_arg, int log2_beta,
                                                         int log2_q, KeySlots ks)
 {
+    // MP > 0: the pattern count is a compile-time constant (unrolled loops)
+    const int m = MP > 0 ? MP : m_arg;
     constexpr int M = Fold<N>::M, T = Fold<N>::T, BC = 2 * L;
-    constexpr int PPT = 2 / CT;                  // bin pairs per thread in the MAC
-    constexpr int NA = (CT == 1 && N <= 2048) ? 2 : 1;   // output rows sharing one root lookup
-    static_assert(CT == 1 || CT == 2, "one or two ciphertexts per workgroup");
-    // two ciphertexts per workgroup: full-period root table (no sign flips;
-    // 128 KiB of LDS in all at N = 2048, still one workgroup per CU)
-    constexpr bool FULL = CT == 2 && N <= 2048;
+    constexpr int NR = 2 / RG;                   // accumulator rows per thread group
+    constexpr int PPT = 2 / (CT * RG);           // bin pairs per thread in the MAC
+    constexpr int NA = N <= 2048 ? 2 : 1;        // output rows sharing one root lookup
+    static_assert((CT == 1 || CT == 2) && (RG == 1 || RG == 2) && CT * RG <= 2,
+                  "one or two ciphertexts per workgroup, or one split over two row groups");
+    static_assert(RG == 1 || Wave<N>::OK, "row groups need the wave-local transform");
+    // 512-thread workgroups (two ciphertexts, or two row groups): full-period
+    // root table (no sign flips; at most 128 KiB of LDS in all at N = 2048)
+    constexpr bool FULL = CT * RG == 2 && N <= 2048;
     __shared__ cplx buf[CT * 2 * M];             // per ciphertext: two polynomials
     __shared__ cplx ztab[RootTab<N, FULL>::SIZE];
 
-    const int grp = threadIdx.x / T;             // ciphertext slot of this thread (transforms)
-    const int t = threadIdx.x - grp * T;         // thread index within the slot
+    const int grp = threadIdx.x / T;             // thread group (transforms)
+    const int t = threadIdx.x - grp * T;         // thread index within the group
+    const int cs = grp / RG;                     // ciphertext slot of the group
+    const int row0 = (grp - cs * RG) * NR;       // first accumulator row the group owns
     const int u = threadIdx.x;                   // MAC unit
     const int64_t first = (int64_t)blockIdx.x * CT;
     int64_t ct_of[CT];
@@ -137,14 +150,14 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
     }
     fill_root_table<N, FULL>(ztab);
 
-    // accumulator rows in registers: x[c][i] = ct[c*N + t + i*T]
+    // accumulator rows in registers: x[c][i] = ct[(row0 + c)*N + t + i*T]
     constexpr bool T32 = ACC != ACC_64;
     using acc_t = typename std::conditional<T32, uint32_t, uint64_t>::type;
     const int shift = ACC == ACC_HI ? log2_q - 32 : 0;   // register = class >> shift
-    acc_t x[2][8];
-    const int64_t *src = ct_init + ct_of[grp] * ct_stride;
+    acc_t x[NR][8];
+    const int64_t *src = ct_init + ct_of[cs] * ct_stride + (size_t)row0 * N;
 #pragma unroll
-    for (int c = 0; c < 2; c++)
+    for (int c = 0; c < NR; c++)
 #pragma unroll
         for (int i = 0; i < 8; i++)
             x[c][i] = (acc_t)((uint64_t)src[c * N + t + i * T] >> shift);
@@ -167,7 +180,7 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
     const double inv_q = ldexp(1.0, -log2_q);
     const double inv_m = 1.0 / (double)M;
     const size_t step_stride = (size_t)2 * BC * m * M;
-    cplx *mybuf = buf + grp * 2 * M;
+    cplx *mybuf = buf + cs * 2 * M + row0 * M;
 
 #pragma unroll 1
     for (int g = 0; g < n_iter; g++) {
@@ -190,9 +203,9 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
         for (int lev = 0; lev < L; lev++) {
             // ---- digits of level lev (dec_fft.cuh:19-29), fold, forward transform
             const int sh = log2_q - (lev + 1) * log2_beta - shift;
-            cplx v[2][4];
+            cplx v[NR][4];
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
+            for (int c = 0; c < NR; c++) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int lo = (int)(((acc_t)(x[c][r] + offs) >> sh) & dmask) - half_beta;
@@ -201,14 +214,14 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
                 }
             }
             if constexpr (Wave<N>::OK)
-                forward_wave<N, FULL>(v, mybuf, ztab, t);
+                forward_wave<N, FULL, NR>(v, mybuf, ztab, t);
             else
-                forward_pair<N, FULL>(v, mybuf, ztab, t);
+                forward_pair<N, FULL, NR>(v, mybuf, ztab, t);
 
             // ---- external product, level lev (mul_bootstrap.cuh:42-67)
 #pragma unroll 1
             for (int pp = 0; pp < PPT; pp++) {
-                const int r = CT == 1 ? u + pp * T : u;      // r < M/2; partner M-1-r
+                const int r = CT * RG == 1 ? u + pp * T : u;   // r < M/2; partner M-1-r
 #pragma unroll 1
                 for (int a0 = 0; a0 < 2; a0 += NA) {
                     cplx W[NA][2][CT][2];
@@ -270,7 +283,7 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
         __syncthreads();
 #pragma unroll
         for (int pp = 0; pp < PPT; pp++) {
-            const int r = CT == 1 ? u + pp * T : u;
+            const int r = CT * RG == 1 ? u + pp * T : u;
 #pragma unroll
             for (int c = 0; c < CT; c++)
 #pragma unroll
@@ -280,13 +293,13 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
                 }
         }
         __syncthreads();
-        cplx w[2][4];
+        cplx w[NR][4];
         if constexpr (Wave<N>::OK)
-            inverse_wave<N, FULL>(w, mybuf, ztab, t);
+            inverse_wave<N, FULL, NR>(w, mybuf, ztab, t);
         else
-            inverse_pair<N, FULL>(w, mybuf, ztab, t);
+            inverse_pair<N, FULL, NR>(w, mybuf, ztab, t);
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
+        for (int a = 0; a < NR; a++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const cplx z = cmul(w[a][r], zeta_pow<N, FULL>(ztab, -(t + r * T)));
@@ -302,8 +315,8 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
                     hi = rint(hi);
                     lo = lo - q * floor(fma(lo, inv_q, 0.5));
                     hi = hi - q * floor(fma(hi, inv_q, 0.5));
-                    if (ACC == ACC_HI && g == n_iter - 1 && first + grp < batch) {
-                        int64_t *dst = out + (first + grp) * 2 * N + a * N;
+                    if (ACC == ACC_HI && g == n_iter - 1 && first + cs < batch) {
+                        int64_t *dst = out + (first + cs) * 2 * N + (size_t)(row0 + a) * N;
                         dst[t + r * T] = (int64_t)lo;
                         dst[t + (r + 4) * T] = (int64_t)hi;
                     }
@@ -325,12 +338,12 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
         }
     }
 
-    if (ACC != ACC_HI && first + grp < batch) {
-        int64_t *dst = out + (first + grp) * 2 * N;
+    if (ACC != ACC_HI && first + cs < batch) {
+        int64_t *dst = out + (first + cs) * 2 * N + (size_t)row0 * N;
         // T32: centred representative in [-2^(log2_q-1), 2^(log2_q-1))
         const int sh_out = 64 - log2_q;
 #pragma unroll
-        for (int c = 0; c < 2; c++)
+        for (int c = 0; c < NR; c++)
 #pragma unroll
             for (int i = 0; i < 8; i++)
                 dst[c * N + t + i * T] =
@@ -338,38 +351,50 @@ __global__ __launch_bounds__(N / 8 * CT, 2) void br_kernel(const int64_t *__rest
     }
 }
 
-template <int N, int L, int CT>
+template <int N, int L, int CT, int RG, int MP>
+static void launch_acc(hipStream_t stream, const int64_t *ct, int64_t ct_stride, const cplx *bk,
+                       const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m,
+                       int log2_beta, int log2_q, KeySlots ks)
+{
+    const dim3 grid((unsigned)((batch + CT - 1) / CT)), block(N / 8 * CT * RG);
+    if (log2_q <= 32)
+        hipLaunchKernelGGL((br_kernel<N, L, CT, RG, ACC_32, MP>), grid, block, 0, stream, ct,
+                           ct_stride, bk, idx, out, batch, n_iter, m, log2_beta, log2_q, ks);
+    else if (L * log2_beta + 1 <= 32)
+        hipLaunchKernelGGL((br_kernel<N, L, CT, RG, ACC_HI, MP>), grid, block, 0, stream, ct,
+                           ct_stride, bk, idx, out, batch, n_iter, m, log2_beta, log2_q, ks);
+    else
+        hipLaunchKernelGGL((br_kernel<N, L, CT, RG, ACC_64, MP>), grid, block, 0, stream, ct,
+                           ct_stride, bk, idx, out, batch, n_iter, m, log2_beta, log2_q, ks);
+}
+
+template <int N, int L, int CT, int RG>
 static int launch_br(hipStream_t stream, const int64_t *ct, int64_t ct_stride, const double *bsk,
                      const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m,
                      int log2_beta, int log2_q, KeySlots ks)
 {
-    const int64_t blocks = (batch + CT - 1) / CT;
     const cplx *bk = reinterpret_cast<const cplx *>(bsk);
-    if (log2_q <= 32)
-        hipLaunchKernelGGL((br_kernel<N, L, CT, ACC_32>), dim3((unsigned)blocks), dim3(N / 8 * CT),
-                           0, stream, ct, ct_stride, bk, idx, out, batch, n_iter, m, log2_beta,
-                           log2_q, ks);
-    else if (L * log2_beta + 1 <= 32)
-        hipLaunchKernelGGL((br_kernel<N, L, CT, ACC_HI>), dim3((unsigned)blocks), dim3(N / 8 * CT),
-                           0, stream, ct, ct_stride, bk, idx, out, batch, n_iter, m, log2_beta,
-                           log2_q, ks);
+    // collapse 3 (8 patterns, the reference's default) at N = 1024, 2048 runs
+    // with the pattern loop unrolled
+    if ((N == 1024 || N == 2048) && m == 8)
+        launch_acc<N, L, CT, RG, 8>(stream, ct, ct_stride, bk, idx, out, batch, n_iter, m, log2_beta,
+                                    log2_q, ks);
     else
-        hipLaunchKernelGGL((br_kernel<N, L, CT, ACC_64>), dim3((unsigned)blocks), dim3(N / 8 * CT),
-                           0, stream, ct, ct_stride, bk, idx, out, batch, n_iter, m, log2_beta,
-                           log2_q, ks);
+        launch_acc<N, L, CT, RG, 0>(stream, ct, ct_stride, bk, idx, out, batch, n_iter, m, log2_beta,
+                                    log2_q, ks);
     return launch_status("blind rotation");
 }
 
-template <int N, int CT>
+template <int N, int CT, int RG = 1>
 static int dispatch_l(int64_t l, hipStream_t s, const int64_t *ct, int64_t st, const double *bsk,
                       const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m, int lb,
                       int lq, KeySlots ks)
 {
     switch (l) {
-    case 1: return launch_br<N, 1, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
-    case 2: return launch_br<N, 2, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
-    case 3: return launch_br<N, 3, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
-    case 4: return launch_br<N, 4, CT>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
+    case 1: return launch_br<N, 1, CT, RG>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
+    case 2: return launch_br<N, 2, CT, RG>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
+    case 3: return launch_br<N, 3, CT, RG>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
+    case 4: return launch_br<N, 4, CT, RG>(s, ct, st, bsk, idx, out, batch, n_iter, m, lb, lq, ks);
     default: return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported l");
     }
 }
@@ -423,7 +448,10 @@ static int validate(int64_t ct_elems, const double *bsk, const int64_t *bsk_dims
     return BSLSTM_OK;
 }
 
-template <int CT>
+// CT ciphertexts per workgroup; RG = 2 splits one ciphertext's two
+// accumulator rows over two thread groups (N in 512..2048; other degrees run
+// the one-group form, whose arithmetic is the same).
+template <int CT, int RG = 1>
 static int run(void *stream, const int64_t *ct_init, int64_t ct_elems, const double *bsk,
                const int64_t *bsk_dims, const double *all_rot_fft, int64_t rot_rows,
                int64_t rot_cols, const int32_t *indices, int64_t idx_elems, int64_t *output,
@@ -441,9 +469,9 @@ static int run(void *stream, const int64_t *ct_init, int64_t ct_elems, const dou
     switch (degree) {
     case 128: return dispatch_l<128, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
     case 256: return dispatch_l<256, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
-    case 512: return dispatch_l<512, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
-    case 1024: return dispatch_l<1024, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
-    case 2048: return dispatch_l<2048, CT>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
+    case 512: return dispatch_l<512, CT, RG>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
+    case 1024: return dispatch_l<1024, CT, RG>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
+    case 2048: return dispatch_l<2048, CT, RG>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
     default:
         // 4096 x 2 ciphertexts would need all 160 KiB of LDS: one per workgroup
         return dispatch_l<4096, 1>(l, s, ct_init, st, bsk, indices, output, batch, ni, mm, lb, lq, ks);
@@ -488,9 +516,30 @@ extern "C" int bslstm_mul_bootstrap_slots(void *stream, const int64_t *ct_init, 
     for (int i = 0; i < 6; i++)
         stride *= bsk_dims[i];
     const bslstm::KeySlots ks{slots, stride, (int32_t)slot_count};
-    if (form == 1)
+    switch (form) {
+    case 1:
         return bslstm::run<1>(stream, ct_init, ct_elems, bsk_pool, bsk_dims, nullptr, 0, 0, indices,
                               idx_elems, output, batch, log2_beta, log2_q, l, degree, ks);
-    return bslstm::run<2>(stream, ct_init, ct_elems, bsk_pool, bsk_dims, nullptr, 0, 0, indices,
-                          idx_elems, output, batch, log2_beta, log2_q, l, degree, ks);
+    case 2:
+        return bslstm::run<2>(stream, ct_init, ct_elems, bsk_pool, bsk_dims, nullptr, 0, 0, indices,
+                              idx_elems, output, batch, log2_beta, log2_q, l, degree, ks);
+    case 3:
+        return bslstm::run<1, 2>(stream, ct_init, ct_elems, bsk_pool, bsk_dims, nullptr, 0, 0,
+                                 indices, idx_elems, output, batch, log2_beta, log2_q, l, degree, ks);
+    default:
+        return bslstm::fail(BSLSTM_INVALID_ARGUMENT, "form must be 1, 2 or 3");
+    }
+}
+
+extern "C" int bslstm_mul_bootstrap_rows(void *stream, const int64_t *ct_init, int64_t ct_elems,
+                                         const double *bsk, const int64_t bsk_dims[6],
+                                         const double *all_rot_fft, int64_t rot_rows,
+                                         int64_t rot_cols, const int32_t *indices,
+                                         int64_t idx_elems, int64_t *output, int64_t batch,
+                                         int64_t log2_beta, int64_t log2_q, int64_t l,
+                                         int64_t degree)
+{
+    return bslstm::run<1, 2>(stream, ct_init, ct_elems, bsk, bsk_dims, all_rot_fft, rot_rows,
+                             rot_cols, indices, idx_elems, output, batch, log2_beta, log2_q, l,
+                             degree);
 }

@@ -2,8 +2,9 @@
 // length-N polynomial is one N/2-point complex FFT of
 // u_k = (p_k - i p_{k+N/2}) zeta^k (the reference's F_j = p(zeta^(2j+1))
 // convention, csrc/dec_fft.cuh:33-57, up to the bin order given in
-// tfhe_device.h).  A group of T = N/8 threads transforms two polynomials at
-// once; thread t holds points t + r*T (r < 4) of each in registers.
+// tfhe_device.h).  A group of T = N/8 threads transforms NP (one or two)
+// polynomials at once, stored M apart; thread t holds points t + r*T (r < 4)
+// of each in registers.
 //
 //  * forward_pair / inverse_pair: Stockham radix-4 (+ one radix-2 pass when
 //    log2(N/2) is odd) through LDS, natural bin order, any N in 128..4096.
@@ -56,16 +57,16 @@ __device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, c
 // t + r*T, t = thread index within the ciphertext's group); result left in
 // buf[p*M + bin] in natural bin order.  All threads of the workgroup call it
 // (it contains workgroup barriers).
-template <int N, bool FULL = false>
-__device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
+template <int N, bool FULL = false, int NP = 2>
+__device__ __forceinline__ void forward_pair(cplx (&v)[NP][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
     int Ns;
     if constexpr (Fold<N>::ODD) {
         // radix-2, span 1: butterflies j = t (points t, t+2T) and j = t+T (t+T, t+3T)
-        cplx o[2][4];
+        cplx o[NP][4];
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < NP; p++) {
             o[p][0] = cadd(v[p][0], v[p][2]);
             o[p][1] = csub(v[p][0], v[p][2]);
             o[p][2] = cadd(v[p][1], v[p][3]);
@@ -73,7 +74,7 @@ __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const c
         }
         __syncthreads();
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < NP; p++) {
             buf[p * M + 2 * t] = o[p][0];
             buf[p * M + 2 * t + 1] = o[p][1];
             buf[p * M + 2 * (t + T)] = o[p][2];
@@ -83,11 +84,11 @@ __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const c
         Ns = 2;
     } else {
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
             dft4(v[p], false);
         __syncthreads();
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
                 buf[p * M + 4 * t + r] = v[p][r];
@@ -95,20 +96,20 @@ __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const c
         Ns = 4;
     }
     for (; Ns < M; Ns *= 4) {
-        cplx x[2][4];
+        cplx x[NP][4];
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
                 x[p][r] = buf[p * M + t + r * T];
         const int k = t & (Ns - 1);
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
             r4_twiddle<N, FULL>(x[p], k, Ns, false, ztab);
         __syncthreads();
         const int idxD = (t / Ns) * 4 * Ns + k;
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
                 buf[p * M + idxD + r * Ns] = x[p][r];
@@ -118,26 +119,26 @@ __device__ __forceinline__ void forward_pair(cplx (&v)[2][4], cplx *buf, const c
 
 // Inverse (unnormalised) transform of buf[p*M + bin]; the result lands in
 // registers, v[p][r] = point t + r*T.  Called by all threads of the workgroup.
-template <int N, bool FULL = false>
-__device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
+template <int N, bool FULL = false, int NP = 2>
+__device__ __forceinline__ void inverse_pair(cplx (&v)[NP][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
     constexpr int R4_END = Fold<N>::ODD ? M / 2 : M;   // radix-4 passes cover spans up to R4_END/4
     for (int Ns = 1; Ns < R4_END; Ns *= 4) {
-        cplx x[2][4];
+        cplx x[NP][4];
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
                 x[p][r] = buf[p * M + t + r * T];
         const int k = t & (Ns - 1);
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
             r4_twiddle<N, FULL>(x[p], k, Ns, true, ztab);
         if (!Fold<N>::ODD && Ns * 4 == M) {
             // last pass: idxD == t, outputs at t + r*T
 #pragma unroll
-            for (int p = 0; p < 2; p++)
+            for (int p = 0; p < NP; p++)
 #pragma unroll
                 for (int r = 0; r < 4; r++)
                     v[p][r] = x[p][r];
@@ -146,7 +147,7 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const c
         __syncthreads();
         const int idxD = (t / Ns) * 4 * Ns + k;
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
                 buf[p * M + idxD + r * Ns] = x[p][r];
@@ -155,7 +156,7 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[2][4], cplx *buf, const c
     if constexpr (Fold<N>::ODD) {
         // radix-2, span M/2: butterflies j = t and j = t+T, inputs j and j+M/2
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < NP; p++) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int j = t + h * T;
@@ -208,35 +209,36 @@ __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 // (lane l holds points l + 64r).  If FROM_REGS the first pass takes v[p][r]
 // instead of reading LDS.  If TO_REGS (inverse only) the last pass leaves
 // its outputs (points l + 64r) in v instead of LDS.
-template <int N, bool INV, bool FROM_REGS, bool TO_REGS, bool FULL = false>
-__device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *poly0, cplx *poly1, int rg, int l,
+template <int N, bool INV, bool FROM_REGS, bool TO_REGS, bool FULL = false, int NP = 2>
+__device__ __forceinline__ void wave_passes(cplx (&v)[NP][4], cplx *polys, int rg, int l,
                                             const cplx *ztab)
 {
-    // element e of this wave's region rg of polynomial p: reg[p][region_slot(rg, e)]
-    cplx *reg[2] = {poly0, poly1};
+    // element e of this wave's region rg of polynomial p:
+    // polys[p*M + region_slot(rg, e)]
+    constexpr int M = N / 2;
 #pragma unroll
     for (int Ns = 1; Ns < 256; Ns *= 4) {
-        cplx x[2][4];
+        cplx x[NP][4];
         if (FROM_REGS && Ns == 1) {
 #pragma unroll
-            for (int p = 0; p < 2; p++)
+            for (int p = 0; p < NP; p++)
 #pragma unroll
                 for (int r = 0; r < 4; r++)
                     x[p][r] = v[p][r];
         } else {
 #pragma unroll
-            for (int p = 0; p < 2; p++)
+            for (int p = 0; p < NP; p++)
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    x[p][r] = reg[p][region_slot<N>(rg, l + 64 * r)];
+                    x[p][r] = polys[p * M + region_slot<N>(rg, l + 64 * r)];
         }
         const int k = l & (Ns - 1);
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
             r4_twiddle<N, FULL>(x[p], k, Ns, INV, ztab);
         if (TO_REGS && Ns == 64) {
 #pragma unroll
-            for (int p = 0; p < 2; p++)
+            for (int p = 0; p < NP; p++)
 #pragma unroll
                 for (int r = 0; r < 4; r++)
                     v[p][r] = x[p][r];
@@ -245,30 +247,30 @@ __device__ __forceinline__ void wave_passes(cplx (&v)[2][4], cplx *poly0, cplx *
         wave_sync();
         const int idxD = (l / Ns) * 4 * Ns + k;
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
-                reg[p][region_slot<N>(rg, idxD + r * Ns)] = x[p][r];
+                polys[p * M + region_slot<N>(rg, idxD + r * Ns)] = x[p][r];
         wave_sync();
     }
 }
 
 // Forward transform of two polynomials (v[p][r] = point t + r*T) into
 // buf[p*M + phys(bin)].  Called by all threads of the workgroup.
-template <int N, bool FULL = false>
-__device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
+template <int N, bool FULL = false, int NP = 2>
+__device__ __forceinline__ void forward_wave(cplx (&v)[NP][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = N / 2, W = Wave<N>::W;
     const int wv = t >> 6, l = t & 63;
     if constexpr (W == 1) {
         __syncthreads();
-        wave_passes<N, false, true, false, FULL>(v, buf, buf + M, 0, l, ztab);
+        wave_passes<N, false, true, false, FULL, NP>(v, buf, 0, l, ztab);
     } else {
         // cross-wave DIF stage: z_k(n) = (sum_r x_{n + r M/W} w_W^{rk}) * w_M^{nk}
-        cplx z[2][4];
+        cplx z[NP][4];
         int pos[4];
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < NP; p++) {
             if constexpr (W == 4) {
                 cplx y[4] = {v[p][0], v[p][1], v[p][2], v[p][3]};
                 dft4(y, false);                                 // y_k, n = t
@@ -290,31 +292,31 @@ __device__ __forceinline__ void forward_wave(cplx (&v)[2][4], cplx *buf, const c
         }
         __syncthreads();
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int p = 0; p < NP; p++)
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 buf[p * M + pos[k]] = z[p][k];
         __syncthreads();
-        wave_passes<N, false, false, false, FULL>(v, buf, buf + M, wv, l, ztab);
+        wave_passes<N, false, false, false, FULL, NP>(v, buf, wv, l, ztab);
     }
     __syncthreads();
 }
 
 // Inverse (unnormalised) transform of buf[p*M + phys(bin)]; the result lands
 // in registers, v[p][r] = point t + r*T.  Called by all threads.
-template <int N, bool FULL = false>
-__device__ __forceinline__ void inverse_wave(cplx (&v)[2][4], cplx *buf, const cplx *ztab, int t)
+template <int N, bool FULL = false, int NP = 2>
+__device__ __forceinline__ void inverse_wave(cplx (&v)[NP][4], cplx *buf, const cplx *ztab, int t)
 {
     constexpr int M = N / 2, W = Wave<N>::W;
     const int wv = t >> 6, l = t & 63;
     if constexpr (W == 1) {
-        wave_passes<N, true, false, true, FULL>(v, buf, buf + M, 0, l, ztab);
+        wave_passes<N, true, false, true, FULL, NP>(v, buf, 0, l, ztab);
     } else {
-        wave_passes<N, true, false, false, FULL>(v, buf, buf + M, wv, l, ztab);
+        wave_passes<N, true, false, false, FULL, NP>(v, buf, wv, l, ztab);
         __syncthreads();
         // cross-wave DIT stage: x_{n + r M/W} = sum_k (u_k(n) w_M^{-nk}) w_W^{-rk}
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < NP; p++) {
             if constexpr (W == 4) {
                 cplx y[4];
 #pragma unroll

@@ -55,7 +55,8 @@ def prepare_bsk(BSK: torch.Tensor, n: int, collapse: int) -> torch.Tensor:
     return BSK.reshape(shape).permute(3, 0, 2, 1, 4, 5).contiguous()
 
 
-FORMS = ("plain", "batched")
+FORMS = ("plain", "batched", "rows")
+_FORM_CODE = {"plain": 1, "batched": 2, "rows": 3}
 
 # Launch-shape autotuner: the faster form per (degree, l, n_iter, m, log2
 # batch bucket), measured once on the live call (both forms run and give
@@ -74,31 +75,37 @@ def tuned_forms() -> dict:
 
 def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft: torch.Tensor | None,
                   indices: torch.Tensor, log_beta: int, log_q: int, l: int,
-                  form: str = "batched", slots: torch.Tensor | None = None) -> torch.Tensor:
+                  form: str = "batched", slots: torch.Tensor | None = None,
+                  candidates: tuple = FORMS) -> torch.Tensor:
     """The raw custom call of cuTFHE/multiply.py:17-51 (handler
     csrc/mul_bootstrap.cuh:192-243): int64 ct (…,2,N) -> int64 (batch,2,N).
 
     form "plain": one workgroup per ciphertext (the reference's launch
     shape); "batched": two ciphertexts per workgroup sharing every key load;
+    "rows": one ciphertext per workgroup with its two accumulator rows on two
+    thread groups (the latency form for small batches);
     "auto": the autotuner's choice for this shape (timed on first use; inside
     a HIP-graph capture an untuned shape takes "batched").  All forms give
     bit-identical results.
 
     slots (int32, one per ciphertext): bsk_reshaped is then a pool
     (C, n_iter, 2, l, 2, m, N/2) of client keys and ciphertext i uses
-    pool[slots[i]]; in the batched form consecutive pairs must share a slot."""
+    pool[slots[i]]; in the batched form consecutive pairs must share a slot
+    (callers whose pairs may not share one tune over candidates=("plain",
+    "rows"))."""
     if form == "auto":
         dims = tuple(int(d) for d in bsk_reshaped.shape[-6:])
         batch = indices.numel() // (dims[0] * dims[4]) if dims[0] * dims[4] else 0
-        key = (ct_init.shape[-1], l, dims[0], dims[4], _bucket(batch), slots is not None)
+        key = (ct_init.shape[-1], l, dims[0], dims[4], _bucket(batch), slots is not None,
+               tuple(candidates))
         if key in _TUNED:
             form = _TUNED[key]
         elif torch.cuda.is_current_stream_capturing() or batch == 0:
-            form = "batched"
+            form = "batched" if "batched" in candidates else candidates[0]
         else:
             times = {}
             out = None
-            for f in FORMS:
+            for f in candidates:
                 a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 a.record()
                 out = mul_bootstrap(ct_init, bsk_reshaped, all_rot_fft, indices, log_beta, log_q,
@@ -133,12 +140,13 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
         _core.check(_core.lib().bslstm_mul_bootstrap_slots(
             _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr, sl.data_ptr(),
             bsk.shape[0], idx.data_ptr(), idx.numel(), out.data_ptr(), batch, log_beta, log_q, l,
-            degree, 1 if form == "plain" else 2))
+            degree, _FORM_CODE[form]))
         return out
     rot = None if all_rot_fft is None else all_rot_fft.contiguous()
     rot_rows, rot_cols = (0, 0) if rot is None else (rot.shape[-2], rot.shape[-1])
-    entry = _core.lib().bslstm_mul_bootstrap if form == "plain" else \
-        _core.lib().bslstm_mul_bootstrap_batched
+    entry = {"plain": _core.lib().bslstm_mul_bootstrap,
+             "batched": _core.lib().bslstm_mul_bootstrap_batched,
+             "rows": _core.lib().bslstm_mul_bootstrap_rows}[form]
     _core.check(entry(
         _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr,
         _core.ptr(rot), rot_rows, rot_cols, idx.data_ptr(), idx.numel(), out.data_ptr(), batch,

@@ -29,7 +29,7 @@ import numpy as np
 import torch
 
 from cuTFHE import torus
-from cuTFHE.multiply import log2_int, mul_bootstrap
+from cuTFHE.multiply import FORMS, log2_int, mul_bootstrap
 from nn_tfhe.Spike_LSTM import CipherSpikeLSTM
 
 
@@ -121,8 +121,8 @@ class SlotBootstrapper:
         """Key switch + many-LUT bootstrap of rows ordered by sequence.  Each
         sequence's rows are padded to a multiple of 32 for the key switch so
         that no 32-row tile mixes clients; bootstrap pairs share a key when a
-        sequence has an even number of rows (else the one-per-workgroup
-        form runs)."""
+        sequence has an even number of rows (else only the forms with one
+        ciphertext per workgroup are tuned over)."""
         pk, pb = h
         rows, n_in = pk.shape[0], pk.shape[-1]
         S = self.seq_slots.numel()
@@ -145,8 +145,8 @@ class SlotBootstrapper:
         idx = torus.rotation_indices(a_ks, log2_int(p["q"]), p["degree"], self.collapse, self.n_lut)
         lut = torch.stack([torus._i64(encrypted_LUT[0]), torus._i64(encrypted_LUT[1])], dim=1)
         br = mul_bootstrap(lut, self.pool.pools["bsk"], None, idx, log2_int(p["beta_bs"]),
-                           log2_int(p["q"]), p["l_bs"], form="auto" if per_seq % 2 == 0 else "plain",
-                           slots=self._per(rows))
+                           log2_int(p["q"]), p["l_bs"], form="auto", slots=self._per(rows),
+                           candidates=FORMS if per_seq % 2 == 0 else ("plain", "rows"))
         masks, bodies = torus.rotate_extract(br, b_ks, log2_int(p["q"]), self.n_lut)
         return masks, bodies[..., None]
 

@@ -84,6 +84,23 @@ int bslstm_mul_bootstrap_batched(void *stream,
                                  int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree);
 
 /*
+ * Same contract, operands, errors and bit-identical outputs as
+ * bslstm_mul_bootstrap (csrc/mul_bootstrap.cuh:138-243); the latency form:
+ * one ciphertext per workgroup, its two accumulator rows (mask, body) split
+ * over two thread groups of N/8 threads, so each step's transforms and
+ * multiply-accumulate run on twice the threads.  For small batches, where the
+ * launch is shorter than one wave of workgroups.  Degrees 128, 256 and 4096
+ * run the one-group form.
+ */
+int bslstm_mul_bootstrap_rows(void *stream,
+                              const int64_t *ct_init, int64_t ct_elems,
+                              const double *bsk, const int64_t bsk_dims[6],
+                              const double *all_rot_fft, int64_t rot_rows, int64_t rot_cols,
+                              const int32_t *indices, int64_t idx_elems,
+                              int64_t *output, int64_t batch,
+                              int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree);
+
+/*
  * Rotation exponents of the collapsed blind rotation.
  * Replaces the index preparation of cuTFHE/multiply.py:67-79 (an XLA fusion
  * in the reference):
@@ -291,9 +308,10 @@ int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, const int64_t 
  * every launch of a timestep).
  * ------------------------------------------------------------------------ */
 
-/* bslstm_mul_bootstrap(_batched) with bsk_pool (slot_count, n_iter, 2, l, 2,
- * m, N/2) and slots int32 (batch).  form 1 = one ciphertext per workgroup,
- * 2 = two (then ciphertexts 2i and 2i+1 must share a slot). */
+/* bslstm_mul_bootstrap(_batched, _rows) with bsk_pool (slot_count, n_iter,
+ * 2, l, 2, m, N/2) and slots int32 (batch).  form 1 = one ciphertext per
+ * workgroup, 2 = two (then ciphertexts 2i and 2i+1 must share a slot),
+ * 3 = one ciphertext over two row groups; other values: InvalidArgument. */
 int bslstm_mul_bootstrap_slots(void *stream, const int64_t *ct_init, int64_t ct_elems,
                                const double *bsk_pool, const int64_t bsk_dims[6],
                                const int32_t *slots, int64_t slot_count,

@@ -108,6 +108,10 @@ def test_lstm_path_entry_points_reject_on_host():
                                           13, 64, 2, 2048, 2) == bad
     assert lib.bslstm_packing_slots(None, 8, 8, 1, 40, 256, 8, None, 2, 256, 8, 4, 64, 8, 1 << 30,
                                     8) == bad
+    # unknown blind-rotation form
+    assert lib.bslstm_mul_bootstrap_slots(None, 8, 4096, 8, dims, 8, 4, 8, 4 * 234 * 8, 8, 4,
+                                          13, 64, 2, 2048, 4) == bad
+    assert "form must be" in lib.bslstm_last_error().decode()
     # aliasing the key-switch output onto its input is refused
     assert lib.bslstm_key_switch_lwe_i64(None, 8, 16, 8, 8, 4, 2048, 702, 1, 20, 64, 8, 16) == bad
     assert _core.lib().bslstm_packing_workspace_bytes(2, 2048, 2048) == \

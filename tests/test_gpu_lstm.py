@@ -228,7 +228,7 @@ def test_packing_decrypts_to_the_lwe_phases(cuda_device, T):
     assert np.abs(ph[128:].astype(np.float64)).max() < 2.0 ** 52
 
 
-@pytest.mark.parametrize("form", ["plain", "batched"])
+@pytest.mark.parametrize("form", ["plain", "batched", "rows"])
 def test_blind_rotation_at_q64_vs_exact(cuda_device, form):
     """The int64 accumulator path at the reference's LSTM modulus 2^64
     (l = 2, beta = 2^13, c = 3) against the int128 oracle."""

@@ -59,7 +59,7 @@ def test_rotation_indices_bit_exact(cuda_device, mult):
         np.testing.assert_array_equal(got, ref)
 
 
-@pytest.mark.parametrize("form", ["plain", "batched"])
+@pytest.mark.parametrize("form", ["plain", "batched", "rows"])
 @pytest.mark.parametrize("N,L,log_beta", [(128, 1, 10), (128, 4, 7), (256, 2, 8), (512, 3, 7),
                                           (1024, 3, 7), (2048, 3, 7), (2048, 4, 6), (4096, 2, 9)])
 def test_mul_bootstrap_vs_exact_int128(cuda_device, mult, N, L, log_beta, form):
@@ -75,7 +75,7 @@ def test_mul_bootstrap_vs_exact_int128(cuda_device, mult, N, L, log_beta, form):
 
 
 @pytest.mark.parametrize("log_q,L,lb,tol", [(36, 2, 8, 64 * 16), (24, 2, 6, 64), (20, 1, 10, 64)])
-@pytest.mark.parametrize("form", ["plain", "batched"])
+@pytest.mark.parametrize("form", ["plain", "batched", "rows"])
 def test_mul_bootstrap_other_moduli(cuda_device, mult, log_q, L, lb, tol, form):
     """log2 q > 32 runs the 64-bit accumulator, log2 q < 32 the 32-bit one;
     the FP64 deviation from the exact oracle grows with q."""
@@ -94,9 +94,10 @@ def test_mul_bootstrap_other_moduli(cuda_device, mult, log_q, L, lb, tol, form):
 
 
 @pytest.mark.parametrize("N,L,c", [(2048, 3, 3), (1024, 2, 2), (512, 4, 1), (256, 3, 4)])
-def test_plain_and_batched_forms_bit_identical(cuda_device, mult, N, L, c):
-    """Odd batch (a tail workgroup with one ciphertext) and per-ciphertext
-    test polynomials; collapse 1..4 (m = 2..16 patterns)."""
+def test_all_forms_bit_identical(cuda_device, mult, N, L, c):
+    """plain, batched and rows give identical outputs.  Odd batch (a tail
+    workgroup with one ciphertext) and per-ciphertext test polynomials;
+    collapse 1..4 (m = 2..16 patterns)."""
     case = make_case(N=N, n=12 * c, L=L, log_beta=7, collapse=c, B=9, seed=3 * N + c)
     rng = np.random.default_rng(c)
     cts = rng.integers(-2 ** 31, 2 ** 31, size=(9, 2, N))
@@ -104,9 +105,30 @@ def test_plain_and_batched_forms_bit_identical(cuda_device, mult, N, L, c):
     bsk = dev(case.bsk_fft_k)
     r_plain = host(mult.mul_bootstrap(dev(cts), bsk, None, idx, 7, 32, L, form="plain"))
     r_batch = host(mult.mul_bootstrap(dev(cts), bsk, None, idx, 7, 32, L, form="batched"))
+    r_rows = host(mult.mul_bootstrap(dev(cts), bsk, None, idx, 7, 32, L, form="rows"))
     np.testing.assert_array_equal(r_plain, r_batch)
+    np.testing.assert_array_equal(r_plain, r_rows)
     exact = X.blind_rotate(cts, case.bsk_coef_k, host(idx), 7, 32, L)
     assert np.abs(O.centered_mod(r_batch - exact, 32)).max() <= BR_COEF_TOL
+
+
+def test_mul_bootstrap_key_slots_all_forms(cuda_device, mult):
+    """Two clients' keys in one pool: every ciphertext through its slot
+    equals the single-key launch with that client's key, in every form."""
+    c1 = make_case(N=512, n=24, L=3, log_beta=7, B=6, seed=31)
+    c2 = make_case(N=512, n=24, L=3, log_beta=7, B=6, seed=32)
+    rng = np.random.default_rng(33)
+    cts = dev(rng.integers(-2 ** 31, 2 ** 31, size=(6, 2, 512)))
+    idx = dev(rng.integers(0, 1024, size=(6, 8, 8)).astype(np.int32))
+    keys = [dev(c.bsk_fft_k) for c in (c1, c2)]
+    pool = torch.stack(keys)
+    slots = [0, 0, 1, 1, 0, 0]                  # pairs share a slot (batched form)
+    alone = [host(mult.mul_bootstrap(cts, k, None, idx, 7, 32, 3, form="plain")) for k in keys]
+    want = np.stack([alone[s][i] for i, s in enumerate(slots)])
+    for form in ("plain", "batched", "rows"):
+        got = host(mult.mul_bootstrap(cts, pool, None, idx, 7, 32, 3, form=form,
+                                      slots=dev(np.array(slots, np.int32))))
+        np.testing.assert_array_equal(got, want)
 
 
 def test_mul_bootstrap_broadcast_equals_batched(cuda_device, mult):
