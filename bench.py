@@ -33,7 +33,7 @@ FP64_PEAK_TFLOPS = 78.6
 # ciphertexts), from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), the
 # scales read off a 1 GiB torch.neg calibration kernel in the same runs
 # (profiles/br_kernel_2048_3_pbs16384.md).  None for other batch sizes.
-HBM_TRAFFIC_BYTES = {"batched": 101.53e9 + 0.604e9, "plain": 4.001e12 + 0.570e9}
+HBM_TRAFFIC_BYTES = {"batched": 102.04e9 + 1.124e9, "plain": 202.15e9 + 0.604e9}
 
 
 def fp64_flops_per_bootstrap(n, N, L, c):
@@ -182,6 +182,44 @@ def run_gpu(args):
         "forms": forms,
     }
     return res
+
+
+def run_form_sweep():
+    """Blind-rotation time of the three launch forms at several batch sizes, at
+    the batched-PBS parameters (torus32, l=3) and at the LSTM's (q=2^64, l=2,
+    Bg=2^13, n=702): the data the autotuner's choices rest on.  Random keys and
+    inputs (the time does not depend on their values); 3-launch minimum."""
+    import torch
+    sys.path.insert(0, PKG)
+    from cuTFHE.multiply import mul_bootstrap
+    dev = torch.device("cuda:0")
+    N = 2048
+    out = {}
+    for name, n_iter, L, lb, lq, batches in (("torus32 l=3 n=750", 250, 3, 7, 32, (64, 256, 1024)),
+                                             ("q=2^64 l=2 n=702", 234, 2, 13, 64, (32, 64, 128, 1024))):
+        g = torch.Generator().manual_seed(0)
+        bsk = (torch.randn(n_iter, 2, L, 2, 8, N // 2, dtype=torch.complex128, generator=g)
+               * 2.0 ** 30).to(dev)
+        rows = {}
+        for B in batches:
+            ct = torch.randint(-2 ** 31, 2 ** 31, (B, 2, N), dtype=torch.int64, generator=g).to(dev)
+            idx = torch.randint(0, 2 * N, (B, n_iter, 8), dtype=torch.int32, generator=g).to(dev)
+            ms = {}
+            for form in ("plain", "batched", "rows"):
+                mul_bootstrap(ct, bsk, None, idx, lb, lq, L, form=form)
+                ts = []
+                for _ in range(3):
+                    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    a.record()
+                    mul_bootstrap(ct, bsk, None, idx, lb, lq, L, form=form)
+                    b.record()
+                    b.synchronize()
+                    ts.append(a.elapsed_time(b))
+                ms[form] = round(min(ts), 3)
+            rows[str(B)] = ms
+        out[name] = rows
+        del bsk
+    return {"unit": "ms per launch", "by_batch": out}
 
 
 # one encrypted spiking-LSTM layer at the reference's parameters
@@ -579,6 +617,8 @@ def main():
                     help="run the 32-sequence encrypted classifier (0: skip)")
     ap.add_argument("--serving", type=int, default=1,
                     help="run the continuous-batching serving workload (0: skip)")
+    ap.add_argument("--form-sweep", type=int, default=1,
+                    help="time the three blind-rotation forms at several batch sizes (0: skip)")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
     args = ap.parse_args()
     if args.cpu_baseline_child:
@@ -589,6 +629,8 @@ def main():
               f"multi-GPU path", file=sys.stderr)
         sys.exit(2)
     res = run_gpu(args) if args.steps > 0 else {}
+    if args.form_sweep:
+        res["form_sweep"] = run_form_sweep()
     if args.lstm_steps > 0:
         res["lstm"] = run_lstm(args)
     if args.classifier:
