@@ -239,6 +239,7 @@ def run_lstm(args):
     import parameters_crypto as PC
     import setup_crypto as SC
     from nn_tfhe.Spike_LSTM import CipherSpikeLSTM
+    from nn_tfhe.engine import SequenceEngine
     from schemas.RLWE_jax import encrypt
 
     cfg = LSTM_CFG
@@ -293,6 +294,18 @@ def run_lstm(args):
                     "p99_ms": round(float(np.percentile(ms, 99)), 3),
                     "mean_ms": round(float(ms.mean()), 3),
                     "bootstraps_per_s": round(3 * G * S / (ms.mean() / 1e3), 1)}
+        # the same steps replayed from a captured HIP graph (the timestep
+        # engine's path): no host launch gaps between the step's kernels
+        eng = SequenceEngine(lstm, use_graph=True)
+        seqs = [(x[:, i, 0], x[:, i, 1]) for i in range(S)]
+        st: list = []
+        eng.run(seqs, step_times=st)
+        gms = np.array([m for _, m in st[2:]])              # step 0 captures the graph
+        out[key + "_graph"] = {"sequences": S, "steps": len(gms),
+                               "p50_ms": round(float(np.percentile(gms, 50)), 3),
+                               "p99_ms": round(float(np.percentile(gms, 99)), 3),
+                               "mean_ms": round(float(gms.mean()), 3),
+                               "bootstraps_per_s": round(3 * G * S / (gms.mean() / 1e3), 1)}
     from cuTFHE.multiply import tuned_forms
     out["tuned_forms"] = {str(k): v for k, v in tuned_forms().items()}
     return out
