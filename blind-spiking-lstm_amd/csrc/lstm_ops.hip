@@ -144,26 +144,34 @@ __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
 //     (A_ks, B_ks) = sum_lev dec_lev(A') * G[s, lev]   (FFT-domain gadget
 //                                                product with the Galois key)
 //     A <- A + A_ks,  B <- B + B' + B_ks         (:290-311, :335-343)
-// One workgroup of N/8 threads per LWE; the RLWE stays in registers
+// One workgroup per LWE; the RLWE stays in registers
 // (coefficients t + i*N/8), the automorphism permutes it through LDS.
 // gkey is (log2 N, l, 2, N/2) complex in the reference's bin order.
 // ---------------------------------------------------------------------------
 // slots (optional): LWE w uses the Galois key gkey + slots[w] * (log2 N * l *
 // 2 * N/2), clamped to [0, slot_count) (batches that mix clients).
-template <int N>
-__global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__restrict__ lwe_a,
-                                                           const int64_t *__restrict__ lwe_b,
-                                                           const cplx *__restrict__ gkey, int l,
-                                                           int lb, int lq,
-                                                           int64_t *__restrict__ out,
-                                                           const int32_t *__restrict__ slots,
-                                                           int slot_count)
+// RG = 2 (N in 512..2048, launches of fewer LWEs than two per CU, where the
+// latency of the log2 N dependent steps is the cost): two groups of N/8
+// threads per LWE, each keeping one row of the RLWE and transforming every
+// other pair of gadget levels, so a step runs on twice the threads.  RG = 1:
+// one group.
+template <int N, int RG>
+__global__ __launch_bounds__(N / 8 * RG) void trace_pack_kernel(
+    const int64_t *__restrict__ lwe_a, const int64_t *__restrict__ lwe_b,
+    const cplx *__restrict__ gkey, int l, int lb, int lq, int64_t *__restrict__ out,
+    const int32_t *__restrict__ slots, int slot_count)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T, LOGN = __builtin_ctz(N);
-    __shared__ cplx buf[2 * M];
+    static_assert(RG == 1 || (RG == 2 && Wave<N>::OK), "two groups need the wave-local transform");
+    constexpr int NR = 2 / RG;               // rows (A, B) a group keeps in registers
+    __shared__ cplx buf[RG * 2 * M];         // per group: two transform polynomials
     __shared__ cplx ztab[N];
-    int64_t *xb = reinterpret_cast<int64_t *>(buf);   // 2N int64 for the permutation
-    const int t = threadIdx.x;
+    // 2N int64 for the permutation: aliases the last group's buffer, whose
+    // first write after the reads below comes after forward_any's barrier
+    int64_t *xb = reinterpret_cast<int64_t *>(buf + (RG - 1) * 2 * M);
+    const int h = threadIdx.x / T;           // group
+    const int t = threadIdx.x - h * T;
+    cplx *mybuf = buf + h * 2 * M;
     const size_t wid = blockIdx.x;
     if (slots) {
         int sl = slots[wid];
@@ -172,20 +180,26 @@ __global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__rest
     }
     fill_root_table<N>(ztab);
 
-    uint64_t A[8], B[8];
+    // X[c] = row (h*NR + c) of the RLWE: c = 0 is A, 1 is B for RG = 1
+    uint64_t X[NR][8];
     {
         const int64_t *a = lwe_a + wid * N;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int k = t + i * T;
-            const int64_t ak = rhe_shift(tmod((uint64_t)a[k == 0 ? 0 : N - k], lq), LOGN);
-            A[i] = k == 0 ? (uint64_t)ak : 0ULL - (uint64_t)ak;
-            B[i] = 0;
+        for (int c = 0; c < NR; c++) {
+            const bool isA = h * NR + c == 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int k = t + i * T;
+                const int64_t ak = rhe_shift(tmod((uint64_t)a[k == 0 ? 0 : N - k], lq), LOGN);
+                X[c][i] = isA ? (k == 0 ? (uint64_t)ak : 0ULL - (uint64_t)ak) : 0ULL;
+            }
+            if (!isA && t == 0)
+                X[c][0] = (uint64_t)rhe_shift(tmod((uint64_t)lwe_b[wid], lq), LOGN);
         }
-        if (t == 0)
-            B[0] = (uint64_t)rhe_shift(tmod((uint64_t)lwe_b[wid], lq), LOGN);
     }
     const double q = ldexp(1.0, lq), inv_q = ldexp(1.0, -lq);
+    const int pairs = (l + 1) / 2;              // gadget levels are transformed two at a time
+    const int iters = (pairs + RG - 1) / RG;    // forward transforms per group and step
 
 #pragma unroll 1
     for (int st = 0; st < LOGN; st++) {
@@ -193,23 +207,30 @@ __global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__rest
         // ---- automorphism X -> X^k (polynomial_jax.py:171-200)
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int j = t + i * T;
-            const int e = (j * kexp) & (2 * N - 1);
-            const int tg = e & (N - 1);
-            const bool neg = e >= N;
-            xb[tg] = (int64_t)(neg ? 0ULL - A[i] : A[i]);
-            xb[N + tg] = (int64_t)(neg ? 0ULL - B[i] : B[i]);
-        }
-        __syncthreads();
-        int64_t r[8];
-        uint64_t Bp[8];
+        for (int c = 0; c < NR; c++)
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
+            for (int i = 0; i < 8; i++) {
+                const int j = t + i * T;
+                const int e = (j * kexp) & (2 * N - 1);
+                const int tg = e & (N - 1);
+                xb[(h * NR + c) * N + tg] = (int64_t)(e >= N ? 0ULL - X[c][i] : X[c][i]);
+            }
+        __syncthreads();
+        // r: digits source A(X^k); the B row takes B(X^k) now (B <- B + B(X^k)
+        // + B_ks, the sum wraps mod 2^64)
+        int64_t r[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
             r[i] = tmod((uint64_t)xb[t + i * T], lq);
-            Bp[i] = (uint64_t)xb[N + t + i * T];
-        }
-        // ---- gadget product with the Galois key of this step
+#pragma unroll
+        for (int c = 0; c < NR; c++)
+            if (h * NR + c == 1)
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    X[c][i] += (uint64_t)xb[N + t + i * T];
+        // ---- gadget product with the Galois key of this step; group h
+        // transforms the level pairs h, h + RG, ... (every group walks the
+        // whole digit recursion, which is sequential)
         const cplx *key = gkey + (size_t)st * l * 2 * M;
         cplx acc[2][2][2];
 #pragma unroll
@@ -218,20 +239,26 @@ __global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__rest
             for (int pp = 0; pp < 2; pp++)
                 acc[a][pp][0] = acc[a][pp][1] = {0.0, 0.0};
 #pragma unroll 1
-        for (int lev = 0; lev < l; lev += 2) {
+        for (int it = 0; it < iters; it++) {
             cplx v[2][4];
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
-                const int lv = lev + c;
-                double dg[8];
+            for (int g2 = 0; g2 < RG; g2++) {
 #pragma unroll
-                for (int i = 0; i < 8; i++)
-                    dg[i] = lv < l ? (double)next_digit(r[i], lq - (lv + 1) * lb) : 0.0;
+                for (int c = 0; c < 2; c++) {
+                    const int lv = 2 * (it * RG + g2) + c;
+                    double dg[8];
 #pragma unroll
-                for (int rr = 0; rr < 4; rr++)
-                    v[c][rr] = fold_point<N>(dg[rr], dg[rr + 4], t, rr, ztab);
+                    for (int i = 0; i < 8; i++)
+                        dg[i] = lv < l ? (double)next_digit(r[i], lq - (lv + 1) * lb) : 0.0;
+                    if (g2 == h) {
+#pragma unroll
+                        for (int rr = 0; rr < 4; rr++)
+                            v[c][rr] = fold_point<N>(dg[rr], dg[rr + 4], t, rr, ztab);
+                    }
+                }
             }
-            forward_any<N>(v, buf, ztab, t);
+            forward_any<N>(v, mybuf, ztab, t);
+            const int lev = 2 * (it * RG + h);
 #pragma unroll
             for (int pp = 0; pp < 2; pp++) {
                 const int rb = t + pp * T;   // bin pair (rb, M-1-rb) = F-bins 2rb, 2rb+1
@@ -239,8 +266,8 @@ __global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__rest
                 for (int c = 0; c < 2; c++) {
                     if (lev + c >= l)
                         break;
-                    const cplx d0 = buf[c * M + phys<N>(rb)];
-                    const cplx d1 = buf[c * M + phys<N>(M - 1 - rb)];
+                    const cplx d0 = mybuf[c * M + phys<N>(rb)];
+                    const cplx d1 = mybuf[c * M + phys<N>(M - 1 - rb)];
 #pragma unroll
                     for (int a = 0; a < 2; a++) {
                         const cplx *kb = key + ((size_t)(lev + c) * 2 + a) * M;
@@ -251,38 +278,66 @@ __global__ __launch_bounds__(N / 8) void trace_pack_kernel(const int64_t *__rest
             }
         }
         __syncthreads();
+        if constexpr (RG == 1) {
 #pragma unroll
-        for (int pp = 0; pp < 2; pp++) {
-            const int rb = t + pp * T;
+            for (int pp = 0; pp < 2; pp++) {
+                const int rb = t + pp * T;
 #pragma unroll
-            for (int a = 0; a < 2; a++) {
-                buf[a * M + phys<N>(rb)] = acc[a][pp][0];
-                buf[a * M + phys<N>(M - 1 - rb)] = acc[a][pp][1];
+                for (int a = 0; a < 2; a++) {
+                    buf[a * M + phys<N>(rb)] = acc[a][pp][0];
+                    buf[a * M + phys<N>(M - 1 - rb)] = acc[a][pp][1];
+                }
+            }
+        } else {
+            // group h hands its partial product for row 1-h to the other group
+            // (second polynomial of that group's buffer), then sums row h
+            // (component selects, not acc[h]: a per-thread index, or a select
+            // of whole elements, would put acc in memory)
+            const bool g1 = h != 0;
+            auto pick = [](bool sel, const cplx &x1, const cplx &x0) -> cplx {
+                return {sel ? x1.re : x0.re, sel ? x1.im : x0.im};
+            };
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++) {
+                const int rb = t + pp * T;
+                cplx *ob = buf + (1 - h) * 2 * M + M;
+                ob[phys<N>(rb)] = pick(g1, acc[0][pp][0], acc[1][pp][0]);
+                ob[phys<N>(M - 1 - rb)] = pick(g1, acc[0][pp][1], acc[1][pp][1]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++) {
+                const int rb = t + pp * T;
+                const cplx own0 = pick(g1, acc[1][pp][0], acc[0][pp][0]);
+                const cplx own1 = pick(g1, acc[1][pp][1], acc[0][pp][1]);
+                mybuf[phys<N>(rb)] = cadd(own0, mybuf[M + phys<N>(rb)]);
+                mybuf[phys<N>(M - 1 - rb)] = cadd(own1, mybuf[M + phys<N>(M - 1 - rb)]);
             }
         }
         __syncthreads();
-        cplx wv[2][4];
-        inverse_any<N>(wv, buf, ztab, t);
+        cplx wv[NR][4];
+        if constexpr (Wave<N>::OK)
+            inverse_wave<N, false, NR>(wv, mybuf, ztab, t);
+        else
+            inverse_pair<N, false, NR>(wv, mybuf, ztab, t);
 #pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            double lo, hi;
-            unfold_point<N>(wv[0][rr], t, rr, ztab, lo, hi);
-            A[rr] += (uint64_t)f64_to_torus(lo, q, inv_q, lq);
-            A[rr + 4] += (uint64_t)f64_to_torus(hi, q, inv_q, lq);
-            unfold_point<N>(wv[1][rr], t, rr, ztab, lo, hi);
-            B[rr] += Bp[rr] + (uint64_t)f64_to_torus(lo, q, inv_q, lq);
-            B[rr + 4] += Bp[rr + 4] + (uint64_t)f64_to_torus(hi, q, inv_q, lq);
+        for (int c = 0; c < NR; c++) {
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                double lo, hi;
+                unfold_point<N>(wv[c][rr], t, rr, ztab, lo, hi);
+                X[c][rr] += (uint64_t)f64_to_torus(lo, q, inv_q, lq);
+                X[c][rr + 4] += (uint64_t)f64_to_torus(hi, q, inv_q, lq);
+            }
         }
     }
-    int64_t *dst = out + wid * 2 * N;
+    int64_t *dst = out + wid * 2 * N + (size_t)h * NR * N;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        dst[t + i * T] = tmod(A[i], lq);
-        dst[N + t + i * T] = tmod(B[i], lq);
-    }
-}
-
-// ---------------------------------------------------------------------------
+    for (int c = 0; c < NR; c++)
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            dst[c * N + t + i * T] = tmod(X[c][i], lq);
+}// ---------------------------------------------------------------------------
 // Plaintext-LUT products (prepare_LUT, activation.py:103-106, summed over the
 // many-LUT slots as Spike_LSTM.py:129-132 does), for a batch of sequences
 // sharing the LUTs:
@@ -682,10 +737,20 @@ static int pack_lwe_run(void *stream, const int64_t *lwe_a, const int64_t *lwe_b
     if (!lwe_a || !lwe_b || !galois_key_fft || !out)
         return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
     const cplx *gk = reinterpret_cast<const cplx *>(galois_key_fft);
-    BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((trace_pack_kernel<NN>), dim3((unsigned)count),
-                                                    dim3(NN / 8), 0, (hipStream_t)stream, lwe_a,
-                                                    lwe_b, gk, (int)l, (int)log2_beta,
-                                                    (int)log2_q, out, slots, (int)slot_count));
+    const bool two = count < 512 && degree >= 512 && degree <= 2048;
+    if (two) {
+        BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((trace_pack_kernel<NN, Wave<NN>::OK ? 2 : 1>),
+                                                        dim3((unsigned)count),
+                                                        dim3(NN / 8 * (Wave<NN>::OK ? 2 : 1)), 0,
+                                                        (hipStream_t)stream, lwe_a, lwe_b, gk,
+                                                        (int)l, (int)log2_beta, (int)log2_q, out,
+                                                        slots, (int)slot_count));
+    } else {
+        BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((trace_pack_kernel<NN, 1>), dim3((unsigned)count),
+                                                        dim3(NN / 8), 0, (hipStream_t)stream, lwe_a,
+                                                        lwe_b, gk, (int)l, (int)log2_beta,
+                                                        (int)log2_q, out, slots, (int)slot_count));
+    }
     return launch_status("trace packing");
 }
 

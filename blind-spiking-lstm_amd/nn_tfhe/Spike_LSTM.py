@@ -115,6 +115,21 @@ class CipherSpikeLSTM:
         self.lut_i = torch.stack([li[0].reshape(G, -1), li[1].reshape(G, -1)], dim=1)  # (G, 2, N)
         # device-resident exponent of the H rotation (no host copy inside a step)
         self._shift = torch.full((1,), input_dim, dtype=torch.int64, device=dev)
+        self._gates = None      # (weight tensors it was built from, merged weights)
+
+    def _gate_weights(self):
+        """The three gate Linears stacked into one weight set (rows f, i, o)
+        with the union of their supports: one launch computes all three
+        pre-activations (each row's result is the same as its own Linear's).
+        Rebuilt when a Linear's weights are replaced (set_weights)."""
+        lins = (self.W_f, self.W_i, self.W_o)
+        key = tuple((id(l.kernel_polynomial), id(l.bias)) for l in lins)
+        if self._gates is None or self._gates[0] != key:
+            w = torch.cat([l.kernel_polynomial for l in lins])
+            b = torch.cat([l.bias for l in lins])
+            sup = torch.nonzero(w.ne(0).any(dim=0)).reshape(-1).to(torch.int32)
+            self._gates = (key, (w, b, sup))
+        return self._gates[1]
 
     def _lwe_add(self, x, y):
         return (torus.axpy(x[0], y[0], 1, 1, self.log_q), torus.axpy(x[1], y[1], 1, 1, self.log_q))
@@ -136,9 +151,12 @@ class CipherSpikeLSTM:
         C_lwe = (cm.reshape(S, nl, G, N), cb.reshape(S, nl, G))
         # HX = x_t + H; the three gate pre-activations
         HX = torus.axpy(x_t, H, 1, 1, lq)
-        F_t = self.W_f((HX[:, 0], HX[:, 1]))
-        I_t = self.W_i((HX[:, 0], HX[:, 1]))
-        O_t = self.W_o((HX[:, 0], HX[:, 1]))
+        w, bias, sup = self._gate_weights()
+        gm, gb = torus.linear_extract(HX, w, bias, lq, sup, check_support=False)
+        Gf, Gi = self.W_f.kernel_polynomial.shape[0], self.W_i.kernel_polynomial.shape[0]
+        split = (Gf, Gi, gm.shape[1] - Gf - Gi)
+        F_t, I_t, O_t = ((m_, b_.unsqueeze(-1)) for m_, b_ in zip(gm.split(split, dim=1),
+                                                                  gb.split(split, dim=1)))
         # first bootstrap: forget gate with the C-dependent LUT, input gate
         lut_C = self.heavyBS.prepare_LUT_sum(C_lwe, self.lut_f_fft)             # (S, G, 2, N)
         inp = (torch.cat([F_t[0], I_t[0]], dim=1), torch.cat([F_t[1], I_t[1]], dim=1))

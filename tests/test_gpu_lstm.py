@@ -296,6 +296,25 @@ def _build_device_layer(m: _Model):
     return lstm, sk
 
 
+def test_merged_gate_linear_equals_separate(cuda_device):
+    """The step's one-launch gate Linear gives each gate exactly what its own
+    Linear gives, and follows a set_weights replacement."""
+    m = _Model()
+    lstm, _ = _build_device_layer(m)
+    rng = np.random.default_rng(9)
+    hx = dev(rng.integers(-2 ** 63, 2 ** 63, size=(3, 2, m.N), dtype=np.int64))
+    w, b, sup = lstm._gate_weights()
+    from cuTFHE import torus
+    gm, gb = torus.linear_extract(hx, w, b, 64, sup, check_support=False)
+    for k, lin in enumerate((lstm.W_f, lstm.W_i, lstm.W_o)):
+        sm, sb = lin((hx[:, 0], hx[:, 1]))
+        assert torch.equal(gm[:, k * m.G:(k + 1) * m.G], sm)
+        assert torch.equal(gb[:, k * m.G:(k + 1) * m.G], sb[..., 0])
+    lstm.W_i.set_weights(np.zeros((m.input_dim + m.hidden, m.G)), np.ones(m.G))
+    w2, b2, _ = lstm._gate_weights()
+    assert not torch.equal(w2, w) and torch.equal(w2[m.G:2 * m.G], lstm.W_i.kernel_polynomial)
+
+
 def test_lstm_timesteps_decrypt_to_the_plaintext_rule(cuda_device):
     """Three timesteps.  At each step the plaintext rule is evaluated on the
     decrypted previous states (so one unit called near a threshold does not
