@@ -176,7 +176,7 @@ def run_gpu(args):
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": (HBM_TRAFFIC_BYTES[main_form] if B == CFG["B"] else None),
-                     "kernel": f"br_kernel<2048,3,{1 if main_form == 'plain' else 2}>",
+                     "kernel": f"br_kernel<2048,3,{1 if main_form == 'plain' else 2},1,1,8>",
                      "kernel_avg_ms": round(br_avg_ms, 3)},
         "decrypt_ok_sample": ok_frac,
         "forms": forms,
