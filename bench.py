@@ -504,6 +504,100 @@ def cpu_baseline_child(sample: int):
     print(json.dumps({"value": sample / dt, "seconds": dt}))
 
 
+# the reference's CPU-runnable case: 256 LWE ciphertexts, n=630, N=1024, l=3,
+# log2 Bg=7, collapse 3, identity LUT on 3-bit messages with a padding bit,
+# then sample extract and key switch back to n (base 2^4, l=5)
+CFG1 = dict(B=256, n=630, N=1024, L=3, log_beta=7, collapse=3, msg_bits=3, log_q=32,
+            sigma_lwe=2.0 ** 17, sigma_bsk=2.0 ** 7, ks_log_beta=4, ks_l=5, sigma_ks=2.0 ** 10,
+            cpu_sample=256)
+
+
+def run_config1(cpu: bool):
+    """Config-1 PBS (blind rotation, extraction, key switch) of 256 ciphertexts
+    on the GPU through the product path, decrypted and compared with the
+    messages; with cpu=True also the oracle's NumPy restatement of the same
+    computation, timed on one host core (the reference's JAX-CPU path cannot
+    run here).  The oracle also builds the keys and inputs (seeded)."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, PKG)
+    from cuTFHE.multiply import mul_bootstrap, rotation_indices
+    from oracle import tfhe_oracle as O
+    from schemas.LWE_jax import key_switch_LWE
+    from schemas.RLWE_jax import rotate_extract
+
+    c = CFG1
+    B, n, N, L, lb, cl, lq = c["B"], c["n"], c["N"], c["L"], c["log_beta"], c["collapse"], c["log_q"]
+    rng = np.random.default_rng(0)
+    s = rng.integers(0, 2, n)
+    S = rng.integers(0, 2, N)
+    bsk = O.bsk_collapse_coef(rng, s, S, lq, lb, L, cl, c["sigma_bsk"])
+    fft_k = np.ascontiguousarray(O.reshape_bsk_for_kernel(O.bsk_to_fourier_ordered(bsk), n // cl, cl))
+    del bsk
+    ka, kb = O.keyswitch_key(rng, s, S, lq, c["ks_log_beta"], c["ks_l"], c["sigma_ks"])
+    delta = 1 << (lq - c["msg_bits"] - 1)
+    msg = rng.integers(0, 1 << c["msg_bits"], B)
+    a, b = O.lwe_encrypt(rng, msg * delta, s, lq, c["sigma_lwe"])
+    W = N >> c["msg_bits"]
+    j = np.arange(N)
+    T = delta * (((j + W // 2) // W) % (1 << c["msg_bits"]))
+    lut = np.stack([np.zeros(N, np.int64), np.asarray(T, np.int64)])
+
+    dev = torch.device("cuda:0")
+    pk = torch.as_tensor(a.astype(np.float64)).to(dev)
+    bd = torch.as_tensor(b.astype(np.float64)).reshape(B, 1).to(dev)
+    bsk_t = torch.as_tensor(fft_k).to(dev)
+    lut_t = torch.as_tensor(np.ascontiguousarray(np.broadcast_to(lut, (B, 2, N)))).to(dev)
+    ksk = (torch.as_tensor(ka.astype(np.float64)).to(dev),
+           torch.as_tensor(kb.astype(np.float64))[..., None].to(dev))
+    kp = {"beta_ks": 2 ** c["ks_log_beta"], "l_ks": c["ks_l"], "q": 2.0 ** lq}
+
+    def pbs():
+        idx = rotation_indices(pk, lq, N, cl, 1)
+        br = mul_bootstrap(lut_t, bsk_t, None, idx, lb, lq, L, form="auto")
+        m_, b_ = rotate_extract(br, bd, lq, 1)
+        return key_switch_LWE(ksk, (m_[:, 0], b_[:, 0]), kp)
+
+    pbs()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ga, gb = pbs()
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ga = ga.cpu().numpy().astype(np.int64)
+    gb = gb.cpu().numpy().astype(np.int64).reshape(-1)
+    ph = O.centered_mod(gb - ga @ s.astype(np.int64), lq)
+    dec = np.mod(np.rint(ph / delta).astype(np.int64), 1 << (c["msg_bits"] + 1))
+    ms = float(np.median(times))
+    out = {"workload": f"PBS of {B} LWE ciphertexts, n={n}, N={N}, l={L}, log2 Bg={lb}, "
+                       f"collapse {cl}, identity LUT, 3-bit messages, then sample extract and key "
+                       f"switch {N}->{n} (base 2^{c['ks_log_beta']}, l={c['ks_l']})",
+           "gpu": {"ms": round(ms, 3), "bootstraps_per_s": round(B / (ms / 1e3), 1),
+                   "messages_correct": float((dec == msg).mean())}}
+    if cpu:
+        from threadpoolctl import threadpool_limits
+        Bc = c["cpu_sample"]                    # a bounded sample: the first Bc ciphertexts
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            idx = O.rotation_indices(a[:Bc], lq, N, cl, 1)
+            br = O.blind_rotate_f64(np.broadcast_to(lut, (Bc, 2, N)), fft_k, idx, lb, lq, L)
+            m_, b_ = O.cuboot_merge_tail(br, b[:Bc], lq, 1)
+            ra, rb = O.key_switch_lwe(ka, kb, m_[:, 0], b_[:, 0], lq, c["ks_log_beta"], c["ks_l"])
+            dt = time.perf_counter() - t0
+        ph = O.centered_mod(np.asarray(rb, np.int64).reshape(-1) - np.asarray(ra, np.int64) @ s, lq)
+        dec_c = np.mod(np.rint(ph / delta).astype(np.int64), 1 << (c["msg_bits"] + 1))
+        out["cpu_restatement"] = {"sample": Bc, "seconds": round(dt, 2),
+                                  "bootstraps_per_s": round(Bc / dt, 2), "cores": 1,
+                                  "messages_correct": float((dec_c == msg[:Bc]).mean()),
+                                  "gpu_equals_restatement": bool(np.array_equal(dec[:Bc], dec_c))}
+    return out
+
+
 def precision_check(sample: int):
     """Phase error of the GPU blind rotation against the exact int128 oracle
     and against the NumPy FP64 restatement of the reference algorithm, on
@@ -630,6 +724,8 @@ def main():
                     help="run the 32-sequence encrypted classifier (0: skip)")
     ap.add_argument("--serving", type=int, default=1,
                     help="run the continuous-batching serving workload (0: skip)")
+    ap.add_argument("--config1", type=int, default=1,
+                    help="run the 256-ciphertext N=1024 PBS + key switch case (0: skip)")
     ap.add_argument("--form-sweep", type=int, default=1,
                     help="time the three blind-rotation forms at several batch sizes (0: skip)")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
@@ -660,6 +756,8 @@ def main():
     res["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
     if not args.no_cpu_baseline and args.precision_sample > 0:
         res["precision"] = precision_check(args.precision_sample)
+    if args.config1:
+        res["config1"] = run_config1(cpu=not args.no_cpu_baseline)
     print(json.dumps(res), flush=True)
 
 

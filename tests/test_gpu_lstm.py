@@ -390,14 +390,16 @@ def test_engine_uneven_lengths_and_graph_replays_match_eager(cuda_device):
         SequenceEngine(lstm).run(seqs[1:])                  # eager work between replays
 
 
-def test_serving_results_same_alone_and_in_the_mix(cuda_device):
+def test_serving_results_same_alone_and_in_the_mix(cuda_device, tmp_path):
     """Continuous batching across clients: requests from three clients with
     their own device-generated keys (key slots), served together with
     admissions mid-flight and a key pool smaller than the client count (so a
     slot is evicted and reloaded), give bit-identical final states to each
     request served alone; every final cell state decrypts under its own
-    client's key to small multiples of beta_x."""
+    client's key to small multiples of beta_x.  The mixed run reads the keys
+    from client key files, the solo runs from memory."""
     import setup_crypto as SC
+    from nn_tfhe.keystore import DiskKeyProvider, save_client_keys
     from nn_tfhe.serving import EncryptedLSTMServer, KeyPool, Request
     from schemas.RLWE_jax import _phase, encrypt
     m = _Model(seed=2)
@@ -422,16 +424,21 @@ def test_serving_results_same_alone_and_in_the_mix(cuda_device):
         A, b = encrypt(msg, secrets[c][0], m.dp, rng)
         xs.append(torch.stack([dev(A), dev(b)], dim=1))
 
-    def serve(items, slots):
+    for c in range(3):
+        save_client_keys(str(tmp_path / f"client_{c}.safetensors"), keys[c], m.params, c)
+    disk = DiskKeyProvider(str(tmp_path), m.params, device="cuda:0",
+                           ks_bytes=keys[0]["ks"].numel())
+
+    def serve(items, slots, provider=lambda c: keys[c]):
         pool = KeyPool(slots, keys[0], device="cuda:0")
-        srv = EncryptedLSTMServer(args, pool, lambda c: keys[c], max_active=8)
+        srv = EncryptedLSTMServer(args, pool, provider, max_active=8)
         alone = len(items) == 1
         reqs = [Request(i, specs[i][0], xs[i], 0.0 if alone else specs[i][2]) for i in items]
         srv.run(reqs, clock=lambda: float(srv.ticks))           # time counted in ticks
         return {r.rid: (host(r.result[0]), host(r.result[1])) for r in srv.finished}, pool
 
-    mixed, pool = serve(range(4), slots=2)
-    assert pool.loads >= 3                      # three clients through two slots
+    mixed, pool = serve(range(4), slots=2, provider=disk)
+    assert pool.loads >= 3 and disk.reads == pool.loads   # three clients through two slots
     for i in range(4):
         alone, _ = serve([i], slots=1)
         np.testing.assert_array_equal(mixed[i][0], alone[i][0], err_msg=f"H of request {i}")
