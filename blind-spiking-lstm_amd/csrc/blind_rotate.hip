@@ -129,7 +129,11 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
     // 512-thread workgroups (two ciphertexts, or two row groups): full-period
     // root table (no sign flips; at most 128 KiB of LDS in all at N = 2048)
     constexpr bool FULL = CT * RG == 2 && N <= 2048;
-    __shared__ cplx buf[CT * 2 * M];             // per ciphertext: two polynomials
+    // levels transformed per phase: the row-group form transforms two gadget
+    // levels of its row at once (two independent transforms per thread, and
+    // half the phases per step); the others one level of both rows
+    constexpr int LP = RG == 2 ? 2 : 1;
+    __shared__ cplx buf[CT * 2 * M * LP];
     __shared__ cplx ztab[RootTab<N, FULL>::SIZE];
 
     const int grp = threadIdx.x / T;             // thread group (transforms)
@@ -197,6 +201,10 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
     const int line0 = ((int)(blockIdx.x >> 3) % P) * lpw;
     const int line_end = min(lines, line0 + lpw);
     uint32_t warm = 0, pf0 = 0, pf1 = 0;
+    // forward target of this group: RG = 2: row0's two levels [level][M];
+    // RG = 1: both rows of ciphertext cs [row][M].  The product spectra of
+    // ciphertext c sit at buf[c*2M + a*M] for the inverse.
+    cplx *fwd = buf + (RG == 2 ? row0 * 2 * M : cs * 2 * M);
     cplx *mybuf = buf + cs * 2 * M + row0 * M;
 
 #pragma unroll 1
@@ -217,24 +225,43 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
                     prod[a][pp][c][0] = prod[a][pp][c][1] = {0.0, 0.0};
 
 #pragma unroll 1
-        for (int lev = 0; lev < L; lev++) {
-            // ---- digits of level lev (dec_fft.cuh:19-29), fold, forward transform
-            const int sh = log2_q - (lev + 1) * log2_beta - shift;
-            cplx v[NR][4];
+        for (int lev0 = 0; lev0 < L; lev0 += LP) {
+            // ---- digits (dec_fft.cuh:19-29), fold, forward transform: polynomial
+            // k is row k at level lev0 (RG = 1) or row0 at level lev0 + k (RG = 2)
+            cplx v[2][4];
 #pragma unroll
-            for (int c = 0; c < NR; c++) {
+            for (int k = 0; k < 2; k++) {
+                // (past the last level -- the odd-l case, transformed by no one --
+                // the last level's shift keeps the arithmetic defined)
+                const int lev = min(lev0 + (RG == 2 ? k : 0), L - 1);
+                const int c = RG == 2 ? 0 : k;
+                const int sh = log2_q - (lev + 1) * log2_beta - shift;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int lo = (int)(((acc_t)(x[c][r] + offs) >> sh) & dmask) - half_beta;
-                    const int hi = (int)(((acc_t)(x[c][r + 4] + offs) >> sh) & dmask) - half_beta;
-                    v[c][r] = cmul({(double)lo, -(double)hi}, zeta_pow<N, FULL>(ztab, t + r * T));   // (lo - i*hi) * zeta^k
+                    int lo = (int)(((acc_t)(x[c][r] + offs) >> sh) & dmask) - half_beta;
+                    int hi = (int)(((acc_t)(x[c][r + 4] + offs) >> sh) & dmask) - half_beta;
+                    v[k][r] = cmul({(double)lo, -(double)hi}, zeta_pow<N, FULL>(ztab, t + r * T));   // (lo - i*hi) * zeta^k
                 }
             }
-            if constexpr (Wave<N>::OK)
-                forward_wave<N, FULL, NR>(v, mybuf, ztab, t);
-            else
-                forward_pair<N, FULL, NR>(v, mybuf, ztab, t);
+            if (RG == 2 && lev0 + 1 == L) {
+                // odd l: the last phase transforms one level
+                cplx v1[1][4] = {{v[0][0], v[0][1], v[0][2], v[0][3]}};
+                if constexpr (Wave<N>::OK)
+                    forward_wave<N, FULL, 1>(v1, fwd, ztab, t);
+                else
+                    forward_pair<N, FULL, 1>(v1, fwd, ztab, t);
+            } else {
+                if constexpr (Wave<N>::OK)
+                    forward_wave<N, FULL, 2>(v, fwd, ztab, t);
+                else
+                    forward_pair<N, FULL, 2>(v, fwd, ztab, t);
+            }
 
+#pragma unroll 1
+            for (int l2 = 0; l2 < LP; l2++) {
+            const int lev = lev0 + l2;
+            if (lev >= L)
+                break;
             // ---- external product, level lev (mul_bootstrap.cuh:42-67)
 #pragma unroll 1
             for (int pp = 0; pp < PPT; pp++) {
@@ -269,8 +296,9 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
                             acc[a][c][0] = acc[a][c][1] = {0.0, 0.0};
 #pragma unroll
                         for (int b2 = 0; b2 < 2; b2++) {
-                            const cplx d0 = buf[c * 2 * M + b2 * M + phys<N>(r)];
-                            const cplx d1 = buf[c * 2 * M + b2 * M + phys<N>(M - 1 - r)];
+                            const int dofs = RG == 2 ? b2 * 2 * M + l2 * M : c * 2 * M + b2 * M;
+                            const cplx d0 = buf[dofs + phys<N>(r)];
+                            const cplx d1 = buf[dofs + phys<N>(M - 1 - r)];
 #pragma unroll
                             for (int a = 0; a < NA; a++) {
                                 acc[a][c][0] = cfma(W[a][b2][c][0], d0, acc[a][c][0]);
@@ -293,6 +321,7 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
                                         prod[aa + a][qq][c][1] = cadd(prod[aa + a][qq][c][1], acc[a][c][1]);
                                     }
                 }
+            }
             }
         }
 

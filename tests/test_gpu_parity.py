@@ -93,7 +93,8 @@ def test_mul_bootstrap_other_moduli(cuda_device, mult, log_q, L, lb, tol, form):
     assert got.min() >= -half and got.max() < half
 
 
-@pytest.mark.parametrize("N,L,c", [(2048, 3, 3), (1024, 2, 2), (512, 4, 1), (256, 3, 4)])
+@pytest.mark.parametrize("N,L,c", [(2048, 3, 3), (1024, 2, 2), (512, 4, 1), (256, 3, 4),
+                                   (512, 1, 3)])
 def test_all_forms_bit_identical(cuda_device, mult, N, L, c):
     """plain, batched and rows give identical outputs.  Odd batch (a tail
     workgroup with one ciphertext) and per-ciphertext test polynomials;
