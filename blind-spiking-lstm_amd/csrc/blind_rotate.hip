@@ -458,7 +458,7 @@ static int dispatch_l(int64_t l, hipStream_t s, const int64_t *ct, int64_t st, c
     }
 }
 
-// Host-side validation shared by both forms (mul_bootstrap.cuh:138-228 plus
+// Host-side validation shared by all forms (mul_bootstrap.cuh:138-228 plus
 // the shape checks the reference does not make).
 static int validate(int64_t ct_elems, const double *bsk, const int64_t *bsk_dims,
                     const double *all_rot_fft, int64_t rot_rows, int64_t rot_cols,

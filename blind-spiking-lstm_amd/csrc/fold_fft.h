@@ -53,7 +53,7 @@ __device__ __forceinline__ void r4_twiddle(cplx x[4], int k, int Ns, bool inv, c
     dft4(x, inv);
 }
 
-// Forward transform of two polynomials held in registers (v[p][r] = point
+// Forward transform of NP polynomials held in registers (v[p][r] = point
 // t + r*T, t = thread index within the ciphertext's group); result left in
 // buf[p*M + bin] in natural bin order.  All threads of the workgroup call it
 // (it contains workgroup barriers).
@@ -255,7 +255,7 @@ __device__ __forceinline__ void wave_passes(cplx (&v)[NP][4], cplx *polys, int r
     }
 }
 
-// Forward transform of two polynomials (v[p][r] = point t + r*T) into
+// Forward transform of NP polynomials (v[p][r] = point t + r*T) into
 // buf[p*M + phys(bin)].  Called by all threads of the workgroup.
 template <int N, bool FULL = false, int NP = 2>
 __device__ __forceinline__ void forward_wave(cplx (&v)[NP][4], cplx *buf, const cplx *ztab, int t)

@@ -58,9 +58,10 @@ def prepare_bsk(BSK: torch.Tensor, n: int, collapse: int) -> torch.Tensor:
 FORMS = ("plain", "batched", "rows")
 _FORM_CODE = {"plain": 1, "batched": 2, "rows": 3}
 
-# Launch-shape autotuner: the faster form per (degree, l, n_iter, m, log2
-# batch bucket), measured once on the live call (both forms run and give
-# bit-identical results, so the measurement costs one extra launch).
+# Launch-shape autotuner: the fastest form per (degree, l, n_iter, m, log2
+# batch bucket, key slots, candidates), measured once on the live call (the
+# candidate forms run and give bit-identical results, so the measurement costs
+# one extra launch per candidate).
 _TUNED: dict = {}
 
 
