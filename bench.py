@@ -473,7 +473,8 @@ def run_serving(args):
             "p50_latency_per_timestep_ms": round(float(np.percentile(per_step, 50)) * 1e3, 2),
             "p99_latency_per_timestep_ms": round(float(np.percentile(per_step, 99)) * 1e3, 2),
             "bootstraps_per_s": round(srv.bootstraps / wall, 1), "wall_s": round(wall, 2),
-            "ticks": srv.ticks, "key_loads": pool.loads,
+            "ticks": srv.ticks, "gpu_busy_frac": round(srv.gpu_busy_s() / wall, 3),
+            "key_loads": pool.loads,
             "key_bytes_per_client": pool.slot_bytes, "key_pool_bytes": pool.footprint_bytes(),
             "client_keygen_s": round(keygen_s, 2)}
 

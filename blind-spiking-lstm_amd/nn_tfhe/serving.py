@@ -196,6 +196,7 @@ class EncryptedLSTMServer:
         self.finished: list = []
         self.ticks = 0
         self.bootstraps = 0
+        self.tick_events: list = []     # (start, end) HIP events of every tick
 
     def submit(self, req: Request):
         self.waiting.append(req)
@@ -225,6 +226,8 @@ class EncryptedLSTMServer:
         if not self.active:
             return 0
         act = self.active
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
         H = torch.stack([r.H for r in act])
         C = torch.stack([r.C for r in act])
         X = torch.stack([r.x[r.t] for r in act])
@@ -243,28 +246,57 @@ class EncryptedLSTMServer:
             else:
                 keep.append(r)
         self.active = keep
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        self.tick_events.append((e0, e1))
         return len(act)
+
+    def gpu_busy_s(self) -> float:
+        """Device time of all ticks so far (waits for them)."""
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in self.tick_events) / 1e3
 
     def run(self, requests, clock=time.perf_counter):
         """Serve `requests` (arrival times in seconds from the start) in
-        real time.  Returns the finished requests with start/done times."""
+        real time.  Returns the finished requests with start/done times.
+
+        The host prepares and launches tick k+1 while the device runs tick k
+        (all work is on one stream, so tick k+1 reads tick k's states, and a
+        key slot freed at tick k is overwritten only after it); it waits for
+        tick k before launching tick k+2.  A request's done time is when the
+        host sees its last tick complete (at most one tick late)."""
         for r in sorted(requests, key=lambda r: r.arrival):
             self.submit(r)
         t0 = clock()
+        pending = None              # (event, requests whose last tick it ends) of the previous tick
+
+        def settle():
+            pending[0].synchronize()
+            end = clock() - t0
+            for r in pending[1]:
+                r.done = end
+
         while self.waiting or self.active:
             now = clock() - t0
             if not self.active and self.waiting and min(r.arrival for r in self.waiting) > now:
+                if pending is not None:
+                    settle()
+                    pending = None
+                    continue
                 nxt = min(r.arrival for r in self.waiting)
                 if clock() - t0 == now:                 # a clock that only ticks with work
                     now = nxt
                 else:
                     time.sleep(min(0.01, nxt - now))
                     continue
+            n_done = len(self.finished)
             stepped = self.tick(now)
             if stepped:
-                torch.cuda.synchronize()
-                end = clock() - t0
-                for r in self.finished:
-                    if r.done < 0:
-                        r.done = end
+                ev = torch.cuda.Event()
+                ev.record()
+                if pending is not None:
+                    settle()
+                pending = (ev, self.finished[n_done:])
+        if pending is not None:
+            settle()
         return self.finished
