@@ -461,8 +461,8 @@ __global__ __launch_bounds__(N / 8) void poly_fft_kernel(const int64_t *__restri
 // 1. pack_transpose_kernel: masks -> rows P_j (coalesced tile transpose)
 // 2. packing_mac_kernel: each workgroup transforms the digits of a block of
 //    rows and accumulates digit x key per bin; one partial sum per block
-// 3. packing_finalize_kernel: sums the partials in block order, one inverse
-//    transform per half, rounding, body.
+// 3. packing_reduce_kernel sums the partials in block order across the chip;
+//    packing_finalize_kernel: one inverse transform per half, rounding, body.
 // The sum over rows is rounded once, after the transform domain sum (the
 // reference rounds per row, then sums).
 // ---------------------------------------------------------------------------
@@ -573,6 +573,23 @@ __global__ __launch_bounds__(N / 8) void packing_mac_kernel(const int64_t *__res
             dst[a * M + M - 1 - rb] = acc[a][pp][1];
         }
     }
+}
+
+// Sum of the per-block partial spectra, one thread per (packing, row, bin), in
+// block order (the order packing_finalize_kernel would add them in, so the sum
+// is the same); the launch spreads the reduction over the whole chip.
+__global__ __launch_bounds__(256) void packing_reduce_kernel(const cplx *__restrict__ partial, int NB,
+                                                             int M, cplx *__restrict__ reduced)
+{
+    const int bin = blockIdx.x * 256 + threadIdx.x;
+    if (bin >= M)
+        return;
+    const int a = blockIdx.y;
+    const size_t p = blockIdx.z;
+    cplx sum = {0.0, 0.0};
+    for (int blk = 0; blk < NB; blk++)
+        sum = cadd(sum, partial[((p * NB + blk) * 2 + a) * M + bin]);
+    reduced[(p * 2 + a) * M + bin] = sum;
 }
 
 template <int N>
@@ -867,8 +884,23 @@ static int packing_run(void *stream, const int64_t *masks, const int64_t *bodies
                                                     slots, (int)slot_count));
     if (int st = launch_status("packing MAC"))
         return st;
+    // the per-block partials summed across the chip into the (no longer
+    // needed) transpose area: packs x 2 x N/2 complex fit there whenever
+    // there is more than one block (n_in >= 2)
+    const cplx *spec = partial;
+    int spec_blocks = nb;
+    if (nb > 1) {
+        cplx *reduced = reinterpret_cast<cplx *>(rows);
+        const int M = N / 2;
+        hipLaunchKernelGGL(packing_reduce_kernel, dim3((unsigned)((M + 255) / 256), 2, (unsigned)packs),
+                           dim3(256), 0, s, partial, nb, M, reduced);
+        if (int st = launch_status("packing reduce"))
+            return st;
+        spec = reduced;
+        spec_blocks = 1;
+    }
     BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((packing_finalize_kernel<NN>), dim3((unsigned)packs),
-                                                    dim3(NN / 8), 0, s, partial, nb, bodies,
+                                                    dim3(NN / 8), 0, s, spec, spec_blocks, bodies,
                                                     (int)n_lwes, (int)log2_q, out));
     return launch_status("packing finalize");
 }
