@@ -110,6 +110,8 @@ __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
             acc[oo] = 0;
         for (int s = 0; s < K; s++) {
             const int i = support[s];
+            if ((unsigned)i >= (unsigned)N)      // entries outside [0, N) are skipped
+                continue;
             const uint64_t av = (uint64_t)ext[(2 * N - j - i) & (2 * N - 1)];
 #pragma unroll
             for (int oo = 0; oo < LIN_OT; oo++)
@@ -126,6 +128,8 @@ __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
         uint64_t acc = bias ? (uint64_t)bias[o] : 0ULL;
         for (int s = 0; s < K; s++) {
             const int i = support[s];
+            if ((unsigned)i >= (unsigned)N)
+                continue;
             // coefficient 0 of W*B: W_0 B_0 - sum_{i>0} W_i B_{N-i}
             const uint64_t bv = i == 0 ? (uint64_t)Bp[0] : 0ULL - (uint64_t)Bp[N - i];
             acc += (uint64_t)w[(size_t)o * N + i] * bv;

@@ -191,8 +191,8 @@ int bslstm_key_switch_lwe_i64(void *stream, const int64_t *a_in, const int64_t *
  *   ct      int64 (batch, 2, N) or (2, N) broadcast (ct_elems = its size)
  *   weights int64 (out_dim, N) weight polynomials (Linear.set_weights,
  *           Spike_LSTM.py:27-41); support int32 (K) the columns that may be
- *           nonzero in any row (all others are skipped); bias int64 (out_dim)
- *           or NULL
+ *           nonzero in any row (all others are skipped; entries outside
+ *           [0, N) are ignored on the device); bias int64 (out_dim) or NULL
  *   masks   int64 (batch, out_dim, N); bodies int64 (batch, out_dim)
  */
 int bslstm_linear_extract(void *stream, const int64_t *ct, int64_t ct_elems,
