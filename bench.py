@@ -509,13 +509,13 @@ def cpu_baseline_child(sample: int):
 # the reference's CPU-runnable case: 256 LWE ciphertexts, n=630, N=1024, l=3,
 # log2 Bg=7, collapse 3, identity LUT on 3-bit messages with a padding bit,
 # then sample extract and key switch back to n (base 2^4, l=5)
-CFG1 = dict(B=256, n=630, N=1024, L=3, log_beta=7, collapse=3, msg_bits=3, log_q=32,
+PBS_KS_CFG = dict(B=256, n=630, N=1024, L=3, log_beta=7, collapse=3, msg_bits=3, log_q=32,
             sigma_lwe=2.0 ** 17, sigma_bsk=2.0 ** 7, ks_log_beta=4, ks_l=5, sigma_ks=2.0 ** 10,
             cpu_sample=256)
 
 
-def run_config1(cpu: bool):
-    """Config-1 PBS (blind rotation, extraction, key switch) of 256 ciphertexts
+def run_pbs_keyswitch(cpu: bool):
+    """PBS (blind rotation, extraction, key switch) of 256 ciphertexts at N=1024
     on the GPU through the product path, decrypted and compared with the
     messages; with cpu=True also the oracle's NumPy restatement of the same
     computation, timed on one host core (the reference's JAX-CPU path cannot
@@ -529,7 +529,7 @@ def run_config1(cpu: bool):
     from schemas.LWE_jax import key_switch_LWE
     from schemas.RLWE_jax import rotate_extract
 
-    c = CFG1
+    c = PBS_KS_CFG
     B, n, N, L, lb, cl, lq = c["B"], c["n"], c["N"], c["L"], c["log_beta"], c["collapse"], c["log_q"]
     rng = np.random.default_rng(0)
     s = rng.integers(0, 2, n)
@@ -726,7 +726,7 @@ def main():
                     help="run the 32-sequence encrypted classifier (0: skip)")
     ap.add_argument("--serving", type=int, default=1,
                     help="run the continuous-batching serving workload (0: skip)")
-    ap.add_argument("--config1", type=int, default=1,
+    ap.add_argument("--pbs-keyswitch", type=int, default=1,
                     help="run the 256-ciphertext N=1024 PBS + key switch case (0: skip)")
     ap.add_argument("--form-sweep", type=int, default=1,
                     help="time the three blind-rotation forms at several batch sizes (0: skip)")
@@ -758,8 +758,8 @@ def main():
     res["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
     if not args.no_cpu_baseline and args.precision_sample > 0:
         res["precision"] = precision_check(args.precision_sample)
-    if args.config1:
-        res["config1"] = run_config1(cpu=not args.no_cpu_baseline)
+    if args.pbs_keyswitch:
+        res["pbs_keyswitch_n1024"] = run_pbs_keyswitch(cpu=not args.no_cpu_baseline)
     print(json.dumps(res), flush=True)
 
 
