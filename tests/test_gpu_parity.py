@@ -231,9 +231,10 @@ def test_pbs_identity_lut_exhaustive_small(cuda_device):
     assert set(case.msg.tolist()) == set(range(8))
 
 
-def test_pbs_config1_shape_vs_exact(cuda_device):
-    """N=1024, n=630, l=3, log2 Bg=7, identity LUT, 3-bit messages (the CPU
-    baseline configuration), then key switch back to n with base 2^4, l=5."""
+def test_pbs_n1024_n630_identity_then_key_switch_vs_exact(cuda_device):
+    """N=1024, n=630, l=3, log2 Bg=7, identity LUT, 3-bit messages (the
+    reference's CPU-runnable case), then key switch back to n with base 2^4,
+    l=5."""
     from schemas.LWE_jax import key_switch_LWE
     case = make_case(N=1024, n=630, L=3, log_beta=7, B=8, msg_bits=3, seed=7, sigma_bsk=2.0 ** 7)
     masks, bodies = _pbs_gpu(case)
@@ -243,7 +244,7 @@ def test_pbs_config1_shape_vs_exact(cuda_device):
     np.testing.assert_array_equal(dec_g, dec_x)
     np.testing.assert_array_equal(dec_g, expected_mod(case))
     dev_max = np.abs(O.centered_mod(masks.astype(np.int64) - em, 32)).max()
-    print(f"config-1 shape: max|gpu-exact| mask coef {dev_max}; err std gpu {err_g.std():.4g} "
+    print(f"N=1024 n=630: max|gpu-exact| mask coef {dev_max}; err std gpu {err_g.std():.4g} "
           f"exact {err_x.std():.4g}")
     # key switch N=1024 -> n=630 and decrypt under s
     rng = np.random.default_rng(70)
@@ -259,7 +260,7 @@ def test_pbs_config1_shape_vs_exact(cuda_device):
                                   case.msg)
 
 
-def test_pbs_config2_shape_many_lut_vs_exact(cuda_device):
+def test_pbs_n2048_n750_many_lut_vs_exact(cuda_device):
     """N=2048, n=750, l=3, log2 Bg=7, 4-bit signed messages, 4 LUTs per test
     polynomial: decryptions identical to the exact int128 oracle's."""
     case = make_case(N=2048, n=750, L=3, log_beta=7, B=6, n_lut=4, msg_bits=4, seed=8,
@@ -270,7 +271,7 @@ def test_pbs_config2_shape_many_lut_vs_exact(cuda_device):
     dec_x, err_x = decrypt_outputs(case, em, eb)
     np.testing.assert_array_equal(dec_g, dec_x)
     d = np.abs(O.centered_mod(masks.astype(np.int64) - em, 32)).max()
-    print(f"config-2 shape: max|gpu-exact| {d}; err std gpu {err_g.std():.4g} exact {err_x.std():.4g}")
+    print(f"N=2048 n=750: max|gpu-exact| {d}; err std gpu {err_g.std():.4g} exact {err_x.std():.4g}")
     assert d <= BR_COEF_TOL
     # the exact oracle itself meets LUT(m) wherever its modulus-switch error allows
     ok = (dec_x == expected_mod(case))
