@@ -710,7 +710,7 @@ def main():
     ap.add_argument("--form", choices=("plain", "batched", "both"), default="both",
                     help="blind-rotation form(s) to time; the JSON value is the batched form "
                          "unless only plain is asked for")
-    ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--cpu-sample", type=int, default=128)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one elementwise negation over this many MiB (reads and "
                          "writes that many bytes): the calibration kernel for HBM counters")
