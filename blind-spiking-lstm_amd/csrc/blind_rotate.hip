@@ -188,19 +188,25 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
     // L2 warm-up of the next step's key slice (512-thread forms, one workgroup
     // per CU).  Workgroups dispatched to one XCD (blockIdx.x mod 8) run the
     // key in near lockstep; the P resident ones split the slice's 128-byte
-    // lines, and each thread touches one word of up to two lines right after
+    // lines, and each thread touches one word of up to NPF lines right after
     // its multiply-accumulate, so the lines are in L2 when the next step's MAC
     // reads them.  The words are folded into `warm`, which is consumed one
     // step later (the loads are long complete by then) and only stored under a
     // condition that never holds.  (In the 256-thread form the two workgroups
     // per CU drift apart and the extra lines evict each other's slices.)
+    // NPF: the latency form runs few workgroups per XCD (P = 4 at 32
+    // ciphertexts), each then covering up to 2048 lines of a 1 MiB slice.
     constexpr bool WARM = CT * RG == 2;
+    constexpr int NPF = RG == 2 ? 4 : 2;
     const int P = min(32, max(1, (int)((gridDim.x + 7) / 8)));
     const int lines = (int)(step_stride / 8);          // 128-byte lines per key slice
     const int lpw = (lines + P - 1) / P;               // lines per workgroup
     const int line0 = ((int)(blockIdx.x >> 3) % P) * lpw;
     const int line_end = min(lines, line0 + lpw);
-    uint32_t warm = 0, pf0 = 0, pf1 = 0;
+    uint32_t warm = 0, pf[NPF];
+#pragma unroll
+    for (int i = 0; i < NPF; i++)
+        pf[i] = 0;
     // forward target of this group: RG = 2: row0's two levels [level][M];
     // RG = 1: both rows of ciphertext cs [row][M].  The product spectra of
     // ciphertext c sit at buf[c*2M + a*M] for the inverse.
@@ -325,15 +331,19 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
             }
         }
 
-        warm ^= pf0 ^ pf1;
-        pf0 = pf1 = 0;
+#pragma unroll
+        for (int i = 0; i < NPF; i++) {
+            warm ^= pf[i];
+            pf[i] = 0;
+        }
         if (WARM && g + 1 < n_iter) {
             const uint32_t *nxt = reinterpret_cast<const uint32_t *>(bsk + (size_t)(g + 1) * step_stride);
-            const int j0 = line0 + (int)threadIdx.x, j1 = j0 + (int)blockDim.x;
-            if (j0 < line_end)
-                pf0 = nxt[(size_t)j0 * 32];
-            if (j1 < line_end)
-                pf1 = nxt[(size_t)j1 * 32];
+#pragma unroll
+            for (int i = 0; i < NPF; i++) {
+                const int j = line0 + (int)threadIdx.x + i * (int)blockDim.x;
+                if (j < line_end)
+                    pf[i] = nxt[(size_t)j * 32];
+            }
         }
 
         // ---- inverse transforms, untwist, centred mod q, round (mul_bootstrap.cuh:70-93)
@@ -395,7 +405,10 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
         }
     }
 
-    if ((warm ^ pf0 ^ pf1) == 0x9E3779B9u && ks.count < 0)   // ks.count >= 1 always
+#pragma unroll
+    for (int i = 0; i < NPF; i++)
+        warm ^= pf[i];
+    if (warm == 0x9E3779B9u && ks.count < 0)   // ks.count >= 1 always
         out[0] = (int64_t)warm;
     if (ACC != ACC_HI && first + cs < batch) {
         int64_t *dst = out + (first + cs) * 2 * N + (size_t)row0 * N;
