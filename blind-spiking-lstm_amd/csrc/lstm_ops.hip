@@ -256,8 +256,8 @@ __global__ __launch_bounds__(N / 8 * RG) void trace_pack_kernel(
                 for (int i = 0; i < 8; i++)
                     X[c][i] += (uint64_t)xb[N + t + i * T];
         // ---- gadget product with the Galois key of this step; group h
-        // transforms the level pairs h, h + RG, ... (every group walks the
-        // whole digit recursion, which is sequential)
+        // transforms the level pairs h, h + RG, ..., starting the digit
+        // recursion at each pair from residual_at
         const cplx *key = gkey + (size_t)st * l * 2 * M;
         cplx acc[2][2][2];
 #pragma unroll
@@ -268,24 +268,21 @@ __global__ __launch_bounds__(N / 8 * RG) void trace_pack_kernel(
 #pragma unroll 1
         for (int it = 0; it < iters; it++) {
             cplx v[2][4];
-#pragma unroll
-            for (int g2 = 0; g2 < RG; g2++) {
-#pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    const int lv = 2 * (it * RG + g2) + c;
-                    double dg[8];
-#pragma unroll
-                    for (int i = 0; i < 8; i++)
-                        dg[i] = lv < l ? (double)next_digit(r[i], lq - (lv + 1) * lb) : 0.0;
-                    if (g2 == h) {
-#pragma unroll
-                        for (int rr = 0; rr < 4; rr++)
-                            v[c][rr] = fold_point<N>(dg[rr], dg[rr + 4], t, rr, ztab);
-                    }
-                }
-            }
-            forward_any<N>(v, mybuf, ztab, t);
             const int lev = 2 * (it * RG + h);
+            double dg[2][8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                int64_t res = residual_at(r[i], lq - lev * lb, lq);
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+                    dg[c][i] = lev + c < l ? (double)next_digit(res, lq - (lev + c + 1) * lb) : 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++)
+                    v[c][rr] = fold_point<N>(dg[c][rr], dg[c][rr + 4], t, rr, ztab);
+            forward_any<N>(v, mybuf, ztab, t);
 #pragma unroll
             for (int pp = 0; pp < 2; pp++) {
                 const int rb = t + pp * T;   // bin pair (rb, M-1-rb) = F-bins 2rb, 2rb+1

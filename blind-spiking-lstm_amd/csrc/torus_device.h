@@ -41,6 +41,26 @@ __device__ __forceinline__ int64_t next_digit(int64_t &r, int s)
     return d;
 }
 
+// Residual of that recursion before the level whose digit has weight
+// 2^(S - log2_beta), straight from the centred class r (lq bits): with
+// S = log2_q - level*log2_beta it is r's centred remainder mod 2^S, the tie
+// rem = 2^(S-1) going negative when bit S of r is set (the earlier digits'
+// round-half-even never changes the bits below their own weight, and bit S
+// is the parity the last rounding looked at).  S >= lq: r itself; S <= 0:
+// 0 (the recursion has emptied it).  Lets a thread start the recursion at
+// any level.
+__device__ __forceinline__ int64_t residual_at(int64_t r, int S, int lq)
+{
+    if (S >= lq)
+        return r;
+    if (S <= 0)
+        return 0;
+    const uint64_t rem = (uint64_t)r & ((1ULL << S) - 1);
+    const uint64_t half = 1ULL << (S - 1);
+    const bool up = rem > half || (rem == half && (((uint64_t)r >> S) & 1));
+    return up ? (int64_t)(rem - (1ULL << S)) : (int64_t)rem;
+}
+
 // Torus class of a rounded FP64 transform output (jax_inversefourier's
 // round, then centred_mod q).
 __device__ __forceinline__ int64_t f64_to_torus(double v, double q, double inv_q, int lq)

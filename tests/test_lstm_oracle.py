@@ -33,6 +33,50 @@ def test_digits_recompose_at_q64(log_beta, l):
         assert np.array_equal(L.recompose(d, log_beta, 64), x)
 
 
+def _residual_at(x, S, log_q):
+    """The closed form csrc/torus_device.h's residual_at computes: the centred
+    remainder of x mod 2^S, the tie 2^(S-1) negative iff bit S of x is set."""
+    if S >= log_q:
+        return x.copy()
+    if S <= 0:
+        return np.zeros_like(x)
+    u = x.astype(np.uint64)
+    rem = u & np.uint64((1 << S) - 1)
+    half = np.uint64(1 << (S - 1))
+    up = (rem > half) | ((rem == half) & (((u >> np.uint64(S)) & np.uint64(1)) == 1))
+    return np.where(up, (rem - np.uint64(1 << S)).view(np.int64), rem.view(np.int64))
+
+
+@pytest.mark.parametrize("log_q,log_beta,l", [(64, 2, 20), (64, 13, 2), (64, 8, 8), (32, 7, 3),
+                                              (48, 4, 12), (20, 3, 7), (24, 5, 6)])
+def test_digit_recursion_restarts_from_closed_form_residual(log_q, log_beta, l):
+    """The trace-packing kernel starts the digit recursion of each gadget-level
+    pair at level k from the closed-form residual; continuing the recursion
+    from it must give the oracle's digits of levels k, k+1, ..., also where
+    round-half ties are planted at every level (a residual that ignored the
+    tie bit, or kept one bit too few, fails here)."""
+    rng = np.random.default_rng(log_q * 100 + log_beta)
+    x = L.uniform(rng, 4000, log_q).astype(np.uint64)
+    for j in range(l):                        # ties at level j in a slice of the inputs
+        s = log_q - (j + 1) * log_beta
+        if s >= 1:
+            sl = slice(j * 150, j * 150 + 300)
+            x[sl] = (x[sl] & ~np.uint64((1 << s) - 1)) | np.uint64(1 << (s - 1))
+    x = L.tmod(x.view(np.int64), log_q)
+    D = L.digits(x, log_beta, l, log_q)
+    with np.errstate(over="ignore"):
+        for k in range(l):
+            r = _residual_at(x, log_q - k * log_beta, log_q)
+            for j in range(k, l):
+                s = log_q - (j + 1) * log_beta
+                if s <= 0:
+                    d, r = r, np.zeros_like(r)
+                else:
+                    d = L._rhe_shift(r, s)
+                    r = (r.astype(np.uint64) - (d.astype(np.uint64) << np.uint64(s))).view(np.int64)
+                assert np.array_equal(d, D[j]), (k, j)
+
+
 def test_tmod_and_rhe():
     assert L.tmod(np.array([2 ** 31, -2 ** 31 - 1, 5]), 32).tolist() == [-2 ** 31, 2 ** 31 - 1, 5]
     r = np.array([5, 6, 7, -5, -6, -7, 12, -12], np.int64)       # /4: 1.25 1.5 1.75 ...
