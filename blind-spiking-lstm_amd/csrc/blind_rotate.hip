@@ -203,10 +203,7 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
     const int lpw = (lines + P - 1) / P;               // lines per workgroup
     const int line0 = ((int)(blockIdx.x >> 3) % P) * lpw;
     const int line_end = min(lines, line0 + lpw);
-    uint32_t warm = 0, pf[NPF];
-#pragma unroll
-    for (int i = 0; i < NPF; i++)
-        pf[i] = 0;
+    uint32_t warm = 0, pf0 = 0, pf1 = 0, pf2 = 0, pf3 = 0;
     // forward target of this group: RG = 2: row0's two levels [level][M];
     // RG = 1: both rows of ciphertext cs [row][M].  The product spectra of
     // ciphertext c sit at buf[c*2M + a*M] for the inverse.
@@ -331,18 +328,25 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
             }
         }
 
-#pragma unroll
-        for (int i = 0; i < NPF; i++) {
-            warm ^= pf[i];
-            pf[i] = 0;
+        warm ^= pf0 ^ pf1;
+        pf0 = pf1 = 0;
+        if constexpr (NPF == 4) {
+            warm ^= pf2 ^ pf3;
+            pf2 = pf3 = 0;
         }
         if (WARM && g + 1 < n_iter) {
             const uint32_t *nxt = reinterpret_cast<const uint32_t *>(bsk + (size_t)(g + 1) * step_stride);
-#pragma unroll
-            for (int i = 0; i < NPF; i++) {
-                const int j = line0 + (int)threadIdx.x + i * (int)blockDim.x;
-                if (j < line_end)
-                    pf[i] = nxt[(size_t)j * 32];
+            const int j0 = line0 + (int)threadIdx.x, j1 = j0 + (int)blockDim.x;
+            if (j0 < line_end)
+                pf0 = nxt[(size_t)j0 * 32];
+            if (j1 < line_end)
+                pf1 = nxt[(size_t)j1 * 32];
+            if constexpr (NPF == 4) {
+                const int j2 = j1 + (int)blockDim.x, j3 = j2 + (int)blockDim.x;
+                if (j2 < line_end)
+                    pf2 = nxt[(size_t)j2 * 32];
+                if (j3 < line_end)
+                    pf3 = nxt[(size_t)j3 * 32];
             }
         }
 
@@ -405,10 +409,9 @@ __global__ __launch_bounds__(N / 8 * CT * RG, 2) void br_kernel(const int64_t *_
         }
     }
 
-#pragma unroll
-    for (int i = 0; i < NPF; i++)
-        warm ^= pf[i];
-    if (warm == 0x9E3779B9u && ks.count < 0)   // ks.count >= 1 always
+    if constexpr (NPF == 4)
+        warm ^= pf2 ^ pf3;
+    if ((warm ^ pf0 ^ pf1) == 0x9E3779B9u && ks.count < 0)   // ks.count >= 1 always
         out[0] = (int64_t)warm;
     if (ACC != ACC_HI && first + cs < batch) {
         int64_t *dst = out + (first + cs) * 2 * N + (size_t)row0 * N;

@@ -80,10 +80,13 @@ __device__ __forceinline__ void unfold_point(cplx w, int t, int r, const cplx *z
 // product; the reference forms the product with a rounded FP64 FFT
 // (fft_product_twice, polynomial_jax.py:45-54).  Only the weight positions
 // listed in `support` (the nonzero columns, uniform across the launch) are
-// visited.
+// visited.  The weights of the workgroup's rows at a chunk of support
+// positions are staged in LDS first (one broadcast read per position in the
+// loop, instead of a dependent global load per position).
 // ---------------------------------------------------------------------------
 constexpr int LIN_THREADS = 256;
-constexpr int LIN_OT = 8;   // output rows per workgroup
+constexpr int LIN_OT = 8;       // output rows per workgroup
+constexpr int LIN_CHUNK = 256;  // support positions staged per pass
 
 __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
     const int64_t *__restrict__ ct, int64_t ct_stride, const int64_t *__restrict__ w,
@@ -91,8 +94,12 @@ __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
     int lq, int64_t *__restrict__ masks, int64_t *__restrict__ bodies)
 {
     extern __shared__ int64_t ext[];   // 2N: E[v] = A_v (v < N), -A_{v-N}
+    __shared__ uint64_t wst[LIN_CHUNK][LIN_OT];   // W_o[i] of the chunk's positions (0: skipped)
+    __shared__ int32_t sst[LIN_CHUNK];            // the chunk's positions i (0 where skipped)
+    __shared__ uint64_t bst[LIN_CHUNK];           // body coefficient paired with W_o[i]
     const int b = blockIdx.z;
     const int o0 = blockIdx.y * LIN_OT;
+    const bool body_block = blockIdx.x == 0;
     const int64_t *A = ct + (size_t)b * ct_stride;
     const int64_t *Bp = A + N;
     for (int k = threadIdx.x; k < N; k += LIN_THREADS) {
@@ -100,42 +107,51 @@ __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
         ext[k] = a;
         ext[N + k] = (int64_t)(0ULL - (uint64_t)a);
     }
-    __syncthreads();
 
     const int j = blockIdx.x * LIN_THREADS + threadIdx.x;
-    if (j < N) {
-        uint64_t acc[LIN_OT];
+    const int ob = (int)threadIdx.x;   // body row of this thread (body block, ob < LIN_OT)
+    uint64_t acc[LIN_OT];
 #pragma unroll
-        for (int oo = 0; oo < LIN_OT; oo++)
-            acc[oo] = 0;
-        for (int s = 0; s < K; s++) {
-            const int i = support[s];
-            if ((unsigned)i >= (unsigned)N)      // entries outside [0, N) are skipped
-                continue;
-            const uint64_t av = (uint64_t)ext[(2 * N - j - i) & (2 * N - 1)];
+    for (int oo = 0; oo < LIN_OT; oo++)
+        acc[oo] = 0;
+    uint64_t accb = (body_block && ob < LIN_OT && o0 + ob < out_dim && bias) ? (uint64_t)bias[o0 + ob] : 0ULL;
+    for (int s0 = 0; s0 < K; s0 += LIN_CHUNK) {
+        const int cn = min(LIN_CHUNK, K - s0);
+        __syncthreads();   // ext written; the previous chunk consumed
+        for (int ss = threadIdx.x; ss < cn; ss += LIN_THREADS) {
+            const int i = support[s0 + ss];
+            const bool ok = (unsigned)i < (unsigned)N;   // entries outside [0, N) are skipped
+            sst[ss] = ok ? i : 0;
+            // coefficient 0 of W*B: W_0 B_0 - sum_{i>0} W_i B_{N-i}
+            if (body_block)
+                bst[ss] = !ok ? 0ULL : (i == 0 ? (uint64_t)Bp[0] : 0ULL - (uint64_t)Bp[N - i]);
 #pragma unroll
             for (int oo = 0; oo < LIN_OT; oo++)
-                if (o0 + oo < out_dim)
-                    acc[oo] += (uint64_t)w[(size_t)(o0 + oo) * N + i] * av;
+                wst[ss][oo] = ok && o0 + oo < out_dim ? (uint64_t)w[(size_t)(o0 + oo) * N + i] : 0ULL;
         }
+        __syncthreads();
+        if (j < N) {
+#pragma unroll 4
+            for (int ss = 0; ss < cn; ss++) {
+                const uint64_t av = (uint64_t)ext[(2 * N - j - sst[ss]) & (2 * N - 1)];
+#pragma unroll
+                for (int oo = 0; oo < LIN_OT; oo++)
+                    acc[oo] += wst[ss][oo] * av;
+            }
+        }
+        if (body_block && ob < LIN_OT) {
+            for (int ss = 0; ss < cn; ss++)
+                accb += wst[ss][ob] * bst[ss];
+        }
+    }
+    if (j < N) {
 #pragma unroll
         for (int oo = 0; oo < LIN_OT; oo++)
             if (o0 + oo < out_dim)
                 masks[((size_t)b * out_dim + o0 + oo) * N + j] = tmod(acc[oo], lq);
     }
-    if (blockIdx.x == 0 && threadIdx.x < LIN_OT && o0 + (int)threadIdx.x < out_dim) {
-        const int o = o0 + threadIdx.x;
-        uint64_t acc = bias ? (uint64_t)bias[o] : 0ULL;
-        for (int s = 0; s < K; s++) {
-            const int i = support[s];
-            if ((unsigned)i >= (unsigned)N)
-                continue;
-            // coefficient 0 of W*B: W_0 B_0 - sum_{i>0} W_i B_{N-i}
-            const uint64_t bv = i == 0 ? (uint64_t)Bp[0] : 0ULL - (uint64_t)Bp[N - i];
-            acc += (uint64_t)w[(size_t)o * N + i] * bv;
-        }
-        bodies[(size_t)b * out_dim + o] = tmod(acc, lq);
-    }
+    if (body_block && ob < LIN_OT && o0 + ob < out_dim)
+        bodies[(size_t)b * out_dim + o0 + ob] = tmod(accb, lq);
 }
 
 // ---------------------------------------------------------------------------
