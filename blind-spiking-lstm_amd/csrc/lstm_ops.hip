@@ -506,7 +506,7 @@ __global__ __launch_bounds__(N / 8) void poly_fft_kernel(const int64_t *__restri
 // 1. pack_transpose_kernel: masks -> rows P_j (coalesced tile transpose)
 // 2. packing_mac_kernel: each workgroup transforms the digits of a block of
 //    rows and accumulates digit x key per bin; one partial sum per block
-// 3. packing_reduce_kernel sums the partials in block order across the chip;
+// 3. packing_reduce_kernel sums the partials across the chip in a fixed order;
 //    packing_finalize_kernel: one inverse transform per half, rounding, body.
 // The sum over rows is rounded once, after the transform domain sum (the
 // reference rounds per row, then sums).
@@ -620,21 +620,37 @@ __global__ __launch_bounds__(N / 8) void packing_mac_kernel(const int64_t *__res
     }
 }
 
-// Sum of the per-block partial spectra, one thread per (packing, row, bin), in
-// block order (the order packing_finalize_kernel would add them in, so the sum
-// is the same); the launch spreads the reduction over the whole chip.
-__global__ __launch_bounds__(256) void packing_reduce_kernel(const cplx *__restrict__ partial, int NB,
-                                                             int M, cplx *__restrict__ reduced)
+// Sum of the per-block partial spectra across the chip, in an order fixed by
+// NB alone (so a packing gives the same sum in any batch): the blocks are cut
+// into RED_G runs; one thread per (bin, run) adds its run in block order, and
+// the run sums are then added in run order.
+constexpr int RED_G = 8;    // runs of blocks per bin
+constexpr int RED_B = 32;   // bins per workgroup (RED_G * RED_B = 256 threads)
+
+__global__ __launch_bounds__(RED_G * RED_B) void packing_reduce_kernel(const cplx *__restrict__ partial,
+                                                                      int NB, int M,
+                                                                      cplx *__restrict__ reduced)
 {
-    const int bin = blockIdx.x * 256 + threadIdx.x;
-    if (bin >= M)
-        return;
+    __shared__ cplx run_sum[RED_G][RED_B];
+    const int bl = (int)threadIdx.x % RED_B, g = (int)threadIdx.x / RED_B;
+    const int bin = blockIdx.x * RED_B + bl;
     const int a = blockIdx.y;
     const size_t p = blockIdx.z;
+    const int per = (NB + RED_G - 1) / RED_G;
+    const int b0 = g * per, b1 = min(NB, b0 + per);
     cplx sum = {0.0, 0.0};
-    for (int blk = 0; blk < NB; blk++)
-        sum = cadd(sum, partial[((p * NB + blk) * 2 + a) * M + bin]);
-    reduced[(p * 2 + a) * M + bin] = sum;
+    if (bin < M)
+        for (int blk = b0; blk < b1; blk++)
+            sum = cadd(sum, partial[((p * NB + blk) * 2 + a) * M + bin]);
+    run_sum[g][bl] = sum;
+    __syncthreads();
+    if (g == 0 && bin < M) {
+        cplx tot = run_sum[0][bl];
+#pragma unroll
+        for (int r = 1; r < RED_G; r++)
+            tot = cadd(tot, run_sum[r][bl]);
+        reduced[(p * 2 + a) * M + bin] = tot;
+    }
 }
 
 template <int N>
@@ -881,7 +897,10 @@ extern "C" int bslstm_poly_fft(void *stream, const int64_t *polys, int64_t count
     return launch_status("polynomial transform");
 }
 
-static int packing_blocks(int64_t n_in) { return (int)std::min<int64_t>(n_in, 128); }
+// Row blocks of the packing MAC: fixed by n_in alone, never by the number of
+// packings, so that a packing's sum order (and result) does not depend on the
+// batch it runs in.
+static int packing_blocks(int64_t n_in) { return (int)std::min<int64_t>(n_in, 256); }
 
 extern "C" int64_t bslstm_packing_workspace_bytes(int64_t packs, int64_t n_in, int64_t degree)
 {
@@ -937,8 +956,8 @@ static int packing_run(void *stream, const int64_t *masks, const int64_t *bodies
     if (nb > 1) {
         cplx *reduced = reinterpret_cast<cplx *>(rows);
         const int M = N / 2;
-        hipLaunchKernelGGL(packing_reduce_kernel, dim3((unsigned)((M + 255) / 256), 2, (unsigned)packs),
-                           dim3(256), 0, s, partial, nb, M, reduced);
+        hipLaunchKernelGGL(packing_reduce_kernel, dim3((unsigned)((M + RED_B - 1) / RED_B), 2, (unsigned)packs),
+                           dim3(RED_G * RED_B), 0, s, partial, nb, M, reduced);
         if (int st = launch_status("packing reduce"))
             return st;
         spec = reduced;
