@@ -115,4 +115,4 @@ def test_lstm_path_entry_points_reject_on_host():
     # aliasing the key-switch output onto its input is refused
     assert lib.bslstm_key_switch_lwe_i64(None, 8, 16, 8, 8, 4, 2048, 702, 1, 20, 64, 8, 16) == bad
     assert _core.lib().bslstm_packing_workspace_bytes(2, 2048, 2048) == \
-        2 * 2048 * 2048 * 8 + 2 * 128 * 2 * 1024 * 16
+        2 * 2048 * 2048 * 8 + 2 * 256 * 2 * 1024 * 16
