@@ -102,10 +102,10 @@ __global__ void ks_digits_kernel(const int64_t *__restrict__ a_in, int batch, in
         int d = 0;
         if (row < batch && k < K) {
             const int i = k / l, lev = k % l;
-            if (i != cur_i) {   // start the digit recursion of a new input
+            if (i != cur_i) {   // enter the digit recursion of a new input at level lev
                 cur_i = i;
-                r = tmod((uint64_t)a_in[(size_t)row * n_in + i], lq);
-                next_lev = 0;
+                r = residual_at(tmod((uint64_t)a_in[(size_t)row * n_in + i], lq), lq - lev * lb, lq);
+                next_lev = lev;
             }
             int64_t dd = 0;
             while (next_lev <= lev)   // levels arrive in order within one input

@@ -500,7 +500,8 @@ __global__ __launch_bounds__(N / 8) void poly_fft_kernel(const int64_t *__restri
 // Packing of n_lwes LWEs (masks of length n_in) into one RLWE
 // (packing, text_jax.py:211-239):
 //   row j (j < n_in) is the polynomial P_j with coefficient i = a_i[j]
-//   (i < n_lwes, zero above);
+//   (i < n_lwes, zero above; only the n_lwes nonzero coefficients are
+//   stored);
 //   A = -sum_j sum_lev dec_lev(P_j) * KSP[j, lev, 0]
 //   B = (b_0 .. b_{n_lwes-1}, 0 ..) - sum_j sum_lev dec_lev(P_j) * KSP[j, lev, 1]
 // 1. pack_transpose_kernel: masks -> rows P_j (coalesced tile transpose)
@@ -530,8 +531,8 @@ __global__ __launch_bounds__(256) void pack_transpose_kernel(const int64_t *__re
     for (int e = threadIdx.x; e < 64 * 64; e += 256) {
         const int jj = e >> 6, ii = e & 63;
         const int i = i0 + ii, j = j0 + jj;
-        if (j < n_in && i < N)
-            rows[((size_t)p * n_in + j) * N + i] = tile[ii][jj];
+        if (j < n_in && i < n_lwes)
+            rows[((size_t)p * n_in + j) * n_lwes + i] = tile[ii][jj];
     }
 }
 
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(256) void pack_transpose_kernel(const int64_t *__re
 // N/2), clamped to [0, slot_count) (batches that mix clients).
 template <int N>
 __global__ __launch_bounds__(N / 8) void packing_mac_kernel(const int64_t *__restrict__ rows,
-                                                            int n_in, int rows_per_block,
+                                                            int n_lwes, int n_in, int rows_per_block,
                                                             const cplx *__restrict__ ksk, int l,
                                                             int lb, int lq,
                                                             cplx *__restrict__ partial,
@@ -568,11 +569,11 @@ __global__ __launch_bounds__(N / 8) void packing_mac_kernel(const int64_t *__res
             acc[a][pp][0] = acc[a][pp][1] = {0.0, 0.0};
 #pragma unroll 1
     for (int j = j0; j < j1; j++) {
-        const int64_t *row = rows + ((size_t)p * n_in + j) * N;
+        const int64_t *row = rows + ((size_t)p * n_in + j) * n_lwes;
         int64_t r[8];
 #pragma unroll
         for (int i = 0; i < 8; i++)
-            r[i] = row[t + i * T];
+            r[i] = t + i * T < n_lwes ? row[t + i * T] : 0;
 #pragma unroll 1
         for (int lev = 0; lev < l; lev += 2) {
             cplx v[2][4];
@@ -936,14 +937,16 @@ static int packing_run(void *stream, const int64_t *masks, const int64_t *bodies
     const int nb = packing_blocks(n_in);
     const int rpb = (int)((n_in + nb - 1) / nb);
     hipStream_t s = (hipStream_t)stream;
-    dim3 tg((unsigned)((n_in + 63) / 64), (unsigned)(N / 64), (unsigned)packs);
-    hipLaunchKernelGGL(pack_transpose_kernel, tg, dim3(256), 0, s, masks, (int)n_lwes, (int)n_in, N,
-                       (int)log2_q, rows);
-    if (int st = launch_status("packing transpose"))
-        return st;
+    if (n_lwes > 0) {
+        dim3 tg((unsigned)((n_in + 63) / 64), (unsigned)((n_lwes + 63) / 64), (unsigned)packs);
+        hipLaunchKernelGGL(pack_transpose_kernel, tg, dim3(256), 0, s, masks, (int)n_lwes, (int)n_in,
+                           N, (int)log2_q, rows);
+        if (int st = launch_status("packing transpose"))
+            return st;
+    }
     const cplx *kf = reinterpret_cast<const cplx *>(ksk_fft);
     BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((packing_mac_kernel<NN>), dim3(nb, (unsigned)packs),
-                                                    dim3(NN / 8), 0, s, rows, (int)n_in, rpb, kf,
+                                                    dim3(NN / 8), 0, s, rows, (int)n_lwes, (int)n_in, rpb, kf,
                                                     (int)l, (int)log2_beta, (int)log2_q, partial,
                                                     slots, (int)slot_count));
     if (int st = launch_status("packing MAC"))
