@@ -32,9 +32,10 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the blind-rotation kernel at this workload (16384
 # ciphertexts), from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), the
 # scales read off a 1 GiB torch.neg calibration kernel in the same runs
-# (profiles/br_kernel_2048_3_pbs16384.md; batched: mean of two launches).
+# (profiles/br_kernel_2048_3_pbs16384.md; batched: mean of both launches,
+# plain: of the two after the first).
 # None for other batch sizes.
-HBM_TRAFFIC_BYTES = {"batched": 299.86e9 + 1.242e9, "plain": 202.15e9 + 0.604e9}
+HBM_TRAFFIC_BYTES = {"batched": 319.79e9 + 1.242e9, "plain": 202.2e9 + 0.604e9}
 
 
 def fp64_flops_per_bootstrap(n, N, L, c):
