@@ -204,6 +204,92 @@ __global__ __launch_bounds__(64 * KM_WAVES) void ks_mfma_kernel(
     }
 }
 
+// Larger batches with one key: a workgroup covers one 32-column tile and
+// KM4_MT = 4 row tiles; wave w takes row tiles 2*(w/2), 2*(w/2)+1 and limbs
+// 4*(w%2) .. 4*(w%2)+3, so each wave loads 2 digit tiles and 4 key tiles per
+// k step for 8 MFMAs (the kernel above loads 9 for 8) and a key tile serves
+// four row tiles.  The two limb halves of an output meet in LDS before the
+// atomic add.
+constexpr int KM4_MT = 4;
+
+__global__ __launch_bounds__(256) void ks_mfma4_kernel(const int8_t *__restrict__ dtiles,
+                                                       const int8_t *__restrict__ ktiles, int batch,
+                                                       int n_out, int KT, int NT, int MT,
+                                                       int kt_per_split, int64_t *__restrict__ a_out,
+                                                       int64_t *__restrict__ b_out)
+{
+    __shared__ uint64_t upper[2][2][16][64];   // [row-tile pair][tile][register][lane]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int mp = w >> 1, lh = w & 1;
+    const int nt = blockIdx.x;
+    const int mtA = blockIdx.y * KM4_MT + 2 * mp;
+    const int kt0 = blockIdx.z * kt_per_split;
+    const int kt1 = min(KT, kt0 + kt_per_split);
+    v16i acc[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            acc[t][j] = (v16i){0};
+    if (mtA < MT) {   // wave-uniform; a missing second tile re-reads the first, its sums unused
+        const int8_t *ap0 = dtiles + ((size_t)mtA * KT) * 1024 + lane * 16;
+        const int8_t *ap1 = dtiles + ((size_t)min(mtA + 1, MT - 1) * KT) * 1024 + lane * 16;
+        const int8_t *bp = ktiles + (size_t)(4 * lh) * 1024 + lane * 16;
+#pragma unroll 1
+        for (int kt = kt0; kt < kt1; kt++) {
+            const v4i a0 = *reinterpret_cast<const v4i *>(ap0 + (size_t)kt * 1024);
+            const v4i a1 = *reinterpret_cast<const v4i *>(ap1 + (size_t)kt * 1024);
+            const int8_t *bk = bp + ((size_t)kt * NT + nt) * 8 * 1024;
+            v4i bb[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                bb[j] = *reinterpret_cast<const v4i *>(bk + j * 1024);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                acc[0][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bb[j], acc[0][j], 0, 0, 0);
+                acc[1][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bb[j], acc[1][j], 0, 0, 0);
+            }
+        }
+    }
+    // this wave's limbs combined: sum_j acc_j << 8*(4*lh + j) mod 2^64
+    uint64_t part[2][16];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            uint64_t v = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                v += (uint64_t)(int64_t)acc[t][j][r] << (8 * (4 * lh + j));
+            part[t][r] = v;
+        }
+    if (lh == 1)
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                upper[mp][t][r][lane] = part[t][r];
+    __syncthreads();
+    const int n = nt * 32 + (lane & 31);
+    if (lh != 0 || n > n_out)
+        return;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int mt = mtA + t;
+        if (mt >= MT)
+            continue;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int m = mt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (m >= batch)
+                continue;
+            unsigned long long *dst = reinterpret_cast<unsigned long long *>(
+                n < n_out ? a_out + (size_t)m * n_out + n : b_out + m);
+            atomicAdd(dst, (unsigned long long)(part[t][r] + upper[mp][t][r][lane]));
+        }
+    }
+}
+
 static int tiles_k(int64_t n_in, int64_t l) { return (int)((n_in * l + 31) / 32); }
 static int tiles_n(int64_t n_out) { return (int)((n_out + 1 + 31) / 32); }
 
@@ -272,14 +358,26 @@ static int ks_mfma_run(void *stream, const int64_t *a_in, const int64_t *b_in,
     const unsigned eb = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(ks_out_init_kernel, dim3(eb), dim3(256), 0, s, b_in, (int)batch, (int)n_out,
                        a_out, b_out);
-    const int gx = (NT + KM_WAVES - 1) / KM_WAVES;
-    // split K until the grid has ~2 workgroups per CU
-    int ks = (int)std::max<int64_t>(1, std::min<int64_t>(KT, 512 / ((int64_t)gx * MT)));
-    const int kps = (KT + ks - 1) / ks;
-    ks = (KT + kps - 1) / kps;
-    hipLaunchKernelGGL(ks_mfma_kernel, dim3(gx, MT, ks), dim3(64 * KM_WAVES), 0, s, dt, key_tiles,
-                       (int)batch, (int)n_out, KT, NT, kps, tile_slots,
-                       bslstm_ks_mfma_key_bytes(l_ks, n_in, n_out), (int)slot_count, a_out, b_out);
+    if (!tile_slots && MT >= KM4_MT) {
+        // one key, at least four row tiles: four row tiles per workgroup;
+        // split K until the grid has ~4 workgroups per CU
+        const int gy = (MT + KM4_MT - 1) / KM4_MT;
+        int ks = (int)std::max<int64_t>(1, std::min<int64_t>(KT, 1024 / ((int64_t)NT * gy)));
+        const int kps = (KT + ks - 1) / ks;
+        ks = (KT + kps - 1) / kps;
+        hipLaunchKernelGGL(ks_mfma4_kernel, dim3(NT, gy, ks), dim3(256), 0, s, dt, key_tiles,
+                           (int)batch, (int)n_out, KT, NT, MT, kps, a_out, b_out);
+    } else {
+        const int gx = (NT + KM_WAVES - 1) / KM_WAVES;
+        // split K until the grid has ~2 workgroups per CU
+        int ks = (int)std::max<int64_t>(1, std::min<int64_t>(KT, 512 / ((int64_t)gx * MT)));
+        const int kps = (KT + ks - 1) / ks;
+        ks = (KT + kps - 1) / kps;
+        hipLaunchKernelGGL(ks_mfma_kernel, dim3(gx, MT, ks), dim3(64 * KM_WAVES), 0, s, dt,
+                           key_tiles, (int)batch, (int)n_out, KT, NT, kps, tile_slots,
+                           bslstm_ks_mfma_key_bytes(l_ks, n_in, n_out), (int)slot_count, a_out,
+                           b_out);
+    }
     if (int st = launch_status("key switch (MFMA)"))
         return st;
     if (log2_q < 64)
