@@ -78,47 +78,52 @@ __global__ void ks_pack_key_kernel(const int64_t *__restrict__ ksk_a,
     }
 }
 
-__global__ void ks_digits_kernel(const int64_t *__restrict__ a_in, int batch, int n_in, int l,
-                                 int lb, int lq, int KT, int MT, int8_t *__restrict__ tiles)
+// One workgroup per (row tile, run of KD_TILES k tiles): the digits of its 32
+// rows x 32*KD_TILES k are computed into LDS by one thread per (row, input),
+// consecutive threads on consecutive inputs of a row, each entering the
+// digit recursion at its first level in the run; then every thread writes
+// one 16-byte lane block of a tile.
+constexpr int KD_TILES = 4;
+constexpr int KD_K = 32 * KD_TILES;
+
+__global__ __launch_bounds__(256) void ks_digits_kernel(const int64_t *__restrict__ a_in, int batch,
+                                                        int n_in, int l, int lb, int lq, int KT,
+                                                        int MT, int8_t *__restrict__ tiles)
 {
-    // one thread per (mt, kt, lane): 16 consecutive k of one row
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = (int64_t)MT * KT * 64;
-    if (gid >= total)
-        return;
-    const int lane = (int)(gid % 64);
-    const int64_t blk = gid / 64;
-    const int kt = (int)(blk % KT);
-    const int mt = (int)(blk / KT);
-    const int row = mt * 32 + lane % 32;
+    __shared__ int8_t dg[32][KD_K + 16];
+    const int mt = blockIdx.x;
+    const int kt0 = blockIdx.y * KD_TILES;
+    const int kbase = kt0 * 32;
     const int K = n_in * l;
-    const int k0 = kt * 32 + (lane / 32) * 16;
-    int8_t dg[16];
-    int cur_i = -1;
-    int64_t r = 0;
-    int next_lev = 0;
-    for (int j = 0; j < 16; j++) {
-        const int k = k0 + j;
-        int d = 0;
-        if (row < batch && k < K) {
-            const int i = k / l, lev = k % l;
-            if (i != cur_i) {   // enter the digit recursion of a new input at level lev
-                cur_i = i;
-                r = residual_at(tmod((uint64_t)a_in[(size_t)row * n_in + i], lq), lq - lev * lb, lq);
-                next_lev = lev;
-            }
-            int64_t dd = 0;
-            while (next_lev <= lev)   // levels arrive in order within one input
-                dd = next_digit(r, lq - (++next_lev) * lb);
-            d = (int)dd;
-        }
-        dg[j] = (int8_t)d;
+    for (int e = threadIdx.x; e < 32 * (KD_K + 16) / 4; e += 256)
+        reinterpret_cast<int32_t *>(&dg[0][0])[e] = 0;
+    __syncthreads();
+    const int i_lo = kbase / l;
+    const int i_hi = min(n_in - 1, (kbase + KD_K - 1) / l);
+    const int ni = i_hi - i_lo + 1;
+    for (int e = threadIdx.x; e < 32 * ni; e += 256) {
+        const int rr = e / ni, i = i_lo + (e - rr * ni);
+        const int row = mt * 32 + rr;
+        if (row >= batch)
+            continue;
+        const int k_first = max(i * l, kbase);
+        const int k_end = min(min(i * l + l, kbase + KD_K), K);
+        int64_t r = residual_at(tmod((uint64_t)a_in[(size_t)row * n_in + i], lq),
+                                lq - (k_first - i * l) * lb, lq);
+        for (int k = k_first; k < k_end; k++)
+            dg[rr][k - kbase] = (int8_t)next_digit(r, lq - (k - i * l + 1) * lb);
     }
+    __syncthreads();
+    const int tile = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (kt0 + tile >= KT)
+        return;
+    const int8_t *src = &dg[lane & 31][tile * 32 + (lane >> 5) * 16];
     v4i w;
+#pragma unroll
     for (int q = 0; q < 4; q++)
-        w[q] = (int)((uint32_t)(uint8_t)dg[4 * q] | ((uint32_t)(uint8_t)dg[4 * q + 1] << 8) |
-                     ((uint32_t)(uint8_t)dg[4 * q + 2] << 16) | ((uint32_t)(uint8_t)dg[4 * q + 3] << 24));
-    *reinterpret_cast<v4i *>(tiles + ((size_t)mt * KT + kt) * 1024 + lane * 16) = w;
+        w[q] = (int)((uint32_t)(uint8_t)src[4 * q] | ((uint32_t)(uint8_t)src[4 * q + 1] << 8) |
+                     ((uint32_t)(uint8_t)src[4 * q + 2] << 16) | ((uint32_t)(uint8_t)src[4 * q + 3] << 24));
+    *reinterpret_cast<v4i *>(tiles + ((size_t)mt * KT + kt0 + tile) * 1024 + lane * 16) = w;
 }
 
 __global__ void ks_out_init_kernel(const int64_t *__restrict__ b_in, int batch, int n_out,
@@ -339,8 +344,9 @@ static int ks_mfma_run(void *stream, const int64_t *a_in, const int64_t *b_in,
     if (l_ks < 1 || log2_beta_ks < 1 || log2_beta_ks > 7 || l_ks * log2_beta_ks > log2_q)
         return fail(BSLSTM_INVALID_ARGUMENT,
                     "int8 digits need 1 <= log2_beta_ks <= 7 and l_ks*log2_beta_ks <= log2_q");
-    if (bslstm_ks_mfma_key_bytes(l_ks, n_in, n_out) < 0 || batch < 0 || batch > (1LL << 24))
-        return fail(BSLSTM_INVALID_ARGUMENT, "bad shapes (n_in*l_ks < 2^18)");
+    // row tiles index the grid's y dimension (at most 65535)
+    if (bslstm_ks_mfma_key_bytes(l_ks, n_in, n_out) < 0 || batch < 0 || batch > 65535LL * 32)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad shapes (n_in*l_ks < 2^18, batch <= 2097120)");
     if (batch == 0)
         return BSLSTM_OK;
     if (workspace_bytes < bslstm_ks_mfma_workspace_bytes(batch, l_ks, n_in))
@@ -351,9 +357,9 @@ static int ks_mfma_run(void *stream, const int64_t *a_in, const int64_t *b_in,
     hipStream_t s = (hipStream_t)stream;
     const int KT = tiles_k(n_in, l_ks), NT = tiles_n(n_out), MT = (int)((batch + 31) / 32);
     int8_t *dt = reinterpret_cast<int8_t *>(workspace);
-    const int64_t dtot = (int64_t)MT * KT * 64;
-    hipLaunchKernelGGL(ks_digits_kernel, dim3((unsigned)((dtot + 255) / 256)), dim3(256), 0, s, a_in,
-                       (int)batch, (int)n_in, (int)l_ks, (int)log2_beta_ks, (int)log2_q, KT, MT, dt);
+    hipLaunchKernelGGL(ks_digits_kernel, dim3((unsigned)MT, (unsigned)((KT + KD_TILES - 1) / KD_TILES)),
+                       dim3(256), 0, s, a_in, (int)batch, (int)n_in, (int)l_ks, (int)log2_beta_ks,
+                       (int)log2_q, KT, MT, dt);
     const int64_t total = batch * (n_out + 1);
     const unsigned eb = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(ks_out_init_kernel, dim3(eb), dim3(256), 0, s, b_in, (int)batch, (int)n_out,

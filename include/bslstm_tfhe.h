@@ -292,7 +292,8 @@ int bslstm_ks_mfma_prepare_key(void *stream, const int64_t *ksk_a, const int64_t
 int64_t bslstm_ks_mfma_workspace_bytes(int64_t batch, int64_t l, int64_t n_in);
 
 /* a_in int64 (batch, n_in), b_in int64 (batch) -> a_out int64 (batch, n_out),
- * b_out int64 (batch); key_tiles from bslstm_ks_mfma_prepare_key. */
+ * b_out int64 (batch), batch <= 2097120 (65535 row tiles of 32); key_tiles
+ * from bslstm_ks_mfma_prepare_key. */
 int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, const int64_t *b_in,
                                const int8_t *key_tiles, int64_t batch, int64_t n_in,
                                int64_t n_out, int64_t log2_beta_ks, int64_t l_ks, int64_t log2_q,
