@@ -231,10 +231,13 @@ LSTM_CFG = dict(input_dim=128, hidden=128, n_lut=4, seed=0)
 
 
 def run_lstm(args):
-    """Per-timestep latency of CipherSpikeLSTM.step for `batch` sequences:
-    every launch of one step (state extraction, three Linear layers, two LUT
-    preparations, two key switch + many-LUT bootstraps, two packings, the H
-    rotation), inputs resident in HBM.  Returns a dict for the JSON line."""
+    """Per-timestep latency of the encrypted LSTM step for `batch` sequences,
+    steps chained as a sequence runs them (CipherSpikeLSTM.step_pipelined):
+    every launch of one step (three Linear layers, two LUT preparations --
+    the forget gate's for the next step, overlapped with this step's
+    output-gate bootstrap --, two key switch + many-LUT bootstraps, two
+    packings, the H rotation), inputs resident in HBM.  Returns a dict for
+    the JSON line."""
     import numpy as np
     import torch
     sys.path.insert(0, PKG)
@@ -279,15 +282,18 @@ def run_lstm(args):
         x = x.unsqueeze(1).expand(-1, S, -1, -1).contiguous()                       # (T, S, 2, N)
         H = torch.zeros(S, 2, N, dtype=torch.int64, device=dev)
         C = torch.zeros_like(H)
+        # steps chained as a sequence runs them: each also prepares the next
+        # step's forget-gate LUT (overlapped with its output-gate bootstrap)
+        lut = lstm.forget_lut(C)
         for t in range(2):                                                          # warm-up
-            H, C = lstm.step(H, C, x[t])
+            H, C, lut = lstm.step_pipelined(H, C, x[t], lut)
         torch.cuda.synchronize()
         stream = torch.cuda.current_stream()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(steps)]
         for t in range(steps):
             evs[t][0].record(stream)
-            H, C = lstm.step(H, C, x[t + 2])
+            H, C, lut = lstm.step_pipelined(H, C, x[t + 2], lut)
             evs[t][1].record(stream)
         torch.cuda.synchronize()
         ms = np.array([a.elapsed_time(e) for a, e in evs])

@@ -116,6 +116,7 @@ class CipherSpikeLSTM:
         # device-resident exponent of the H rotation (no host copy inside a step)
         self._shift = torch.full((1,), input_dim, dtype=torch.int64, device=dev)
         self._gates = None      # (weight tensors it was built from, merged weights)
+        self._side = None       # side stream of step_pipelined (default priority)
 
     def _gate_weights(self):
         """The three gate Linears stacked into one weight set (rows f, i, o)
@@ -141,14 +142,37 @@ class CipherSpikeLSTM:
         b = bodies[:, lo:hi].permute(0, 2, 1).reshape(bodies.shape[0], -1)
         return m, b
 
+    # The next step's forget-gate LUT goes to a side stream, overlapping the
+    # output gate's bootstrap, when that bootstrap has at most this many
+    # ciphertexts (its launch then leaves most of the chip idle).
+    OVERLAP_MAX_BOOTSTRAPS = 128
+
+    def forget_lut(self, C):
+        """The forget gate's encrypted LUT from the cell state C (S, 2, N)
+        (Spike_LSTM.py:113-132): the hidden-state LWEs of C (coefficients
+        0..hidden-1, ordered k*G + g), trace-packed and multiplied by the
+        plaintext LUTs, summed over the n_lut slots.  Returns (S, G, 2, N)."""
+        S, N = C.shape[0], self.degree
+        cm, cb = torus.rotate_extract(C, None, self.log_q, 1, self.hidden_dim)
+        C_lwe = (cm.reshape(S, self.n_lut, self.G, N), cb.reshape(S, self.n_lut, self.G))
+        return self.heavyBS.prepare_LUT_sum(C_lwe, self.lut_f_fft)
+
     def step(self, H, C, x_t):
         """One timestep (Spike_LSTM.py:108-178) for S sequences.
         H, C, x_t: RLWEs int64 (S, 2, N).  Returns (H_t, C_t) (S, 2, N)."""
+        H_new, C_new, _ = self.step_pipelined(H, C, x_t, self.forget_lut(C), next_lut=False)
+        return H_new, C_new
+
+    def step_pipelined(self, H, C, x_t, lut_C, next_lut: bool = True):
+        """step() with the forget gate's LUT given (forget_lut(C), or the
+        third result of the previous step_pipelined call) and, if next_lut,
+        the next step's LUT returned as well: (H_t, C_t, forget_lut(C_t)).
+        That LUT depends only on the new cell state, so it is prepared while
+        the output gate's bootstrap runs -- on a side stream when that
+        bootstrap is small -- and leaves the sequential path of the next
+        step.  Results are those of step(), bit for bit."""
         S, N = H.shape[0], self.degree
         G, nl, lq = self.G, self.n_lut, self.log_q
-        # hidden-state LWEs of C (coefficients 0..hidden-1), ordered k*G + g
-        cm, cb = torus.rotate_extract(C, None, lq, 1, self.hidden_dim)
-        C_lwe = (cm.reshape(S, nl, G, N), cb.reshape(S, nl, G))
         # HX = x_t + H; the three gate pre-activations
         HX = torus.axpy(x_t, H, 1, 1, lq)
         w, bias, sup = self._gate_weights()
@@ -158,7 +182,6 @@ class CipherSpikeLSTM:
         F_t, I_t, O_t = ((m_, b_.unsqueeze(-1)) for m_, b_ in zip(gm.split(split, dim=1),
                                                                   gb.split(split, dim=1)))
         # first bootstrap: forget gate with the C-dependent LUT, input gate
-        lut_C = self.heavyBS.prepare_LUT_sum(C_lwe, self.lut_f_fft)             # (S, G, 2, N)
         inp = (torch.cat([F_t[0], I_t[0]], dim=1), torch.cat([F_t[1], I_t[1]], dim=1))
         lut = torch.cat([lut_C, self.lut_i.unsqueeze(0).expand(S, G, 2, N)], dim=1)
         bm, bb = self.heavyBS.cuboot((inp[0].reshape(S * 2 * G, N), inp[1].reshape(-1, 1)),
@@ -168,18 +191,39 @@ class CipherSpikeLSTM:
         sI = self._spikes(bm, bb, G, 2 * G)
         sFC = self._spikes(bm, bb, 0, G)
         C_t = self._lwe_add(sFC, sI)                                            # (S, hidden, .)
+
+        def pack_c():
+            Cp = self.heavyBS.packing((C_t[0], C_t[1].unsqueeze(-1)))
+            C_new = torch.stack([Cp[0], Cp[1]], dim=1)
+            return C_new, (self.forget_lut(C_new) if next_lut else None)
+
+        main = torch.cuda.current_stream()
+        side = None
+        if next_lut and S * G <= self.OVERLAP_MAX_BOOTSTRAPS:
+            if self._side is None:
+                self._side = torch.cuda.Stream(device=H.device)
+            side = self._side
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                C_new, lut_next = pack_c()
         # second bootstrap: output gate with the C_t-dependent LUT
         lut_H = self.heavyBS.prepare_LUT_sum((C_t[0].reshape(S, nl, G, N),
                                               C_t[1].reshape(S, nl, G)), self.lut_o_fft)
         hm, hb = self.heavyBS.cuboot((O_t[0].reshape(S * G, N), O_t[1].reshape(-1, 1)),
                                      [lut_H[:, :, 0].reshape(-1, N), lut_H[:, :, 1].reshape(-1, N)])
         H_t = self._spikes(hm.reshape(S, G, nl, N), hb.reshape(S, G, nl), 0, G)
-        # pack both states back into RLWEs; H moves behind the input slots
-        Cp = self.heavyBS.packing((C_t[0], C_t[1].unsqueeze(-1)))
+        if side is None:
+            C_new, lut_next = pack_c()
+        # pack H back into an RLWE; it moves behind the input slots
         Hp = self.heavyBS.packing((H_t[0], H_t[1].unsqueeze(-1)))
-        C_new = torch.stack([Cp[0], Cp[1]], dim=1)
         H_new = torus.rotate_poly(torch.stack([Hp[0], Hp[1]], dim=1), self._shift, lq)
-        return H_new, C_new
+        if side is not None:
+            main.wait_stream(side)
+            if not torch.cuda.is_current_stream_capturing():
+                # made on the side stream, used on this one from here on
+                C_new.record_stream(main)
+                lut_next.record_stream(main)
+        return H_new, C_new, lut_next
 
     def __call__(self, x, seq_len=None):
         """Spike_LSTM.py:103-184: x = (A (T, N), B (T, N)) for one sequence or
@@ -192,10 +236,11 @@ class CipherSpikeLSTM:
         S, T, N = A.shape
         H = torch.zeros(S, 2, N, dtype=torch.int64, device=A.device)
         C = torch.zeros_like(H)
+        lut_C = self.forget_lut(C)
         Hs, Cs = [], []
         for t in range(T):
             x_t = torch.stack([A[:, t], B[:, t]], dim=1)
-            H, C = self.step(H, C, x_t)
+            H, C, lut_C = self.step_pipelined(H, C, x_t, lut_C, next_lut=t + 1 < T)
             Hs.append(H)
             Cs.append(C)
         Hs, Cs = torch.stack(Hs, dim=1), torch.stack(Cs, dim=1)                # (S, T, 2, N)

@@ -8,7 +8,10 @@ over the prefix puts all their bootstraps into the same batched launches,
 and finished sequences drop out.  Each step can be replayed from a HIP graph
 captured once per live-batch size (static input/state buffers, the step's
 launches recorded on the capture stream), which removes the host launch
-overhead of the ~40 launches per step.
+overhead of the ~40 launches per step.  Steps are chained through
+CipherSpikeLSTM.step_pipelined: each step also prepares the next step's
+forget-gate LUT (kept in a static buffer beside H and C), overlapped with
+its own output-gate bootstrap.
 """
 from __future__ import annotations
 
@@ -30,27 +33,34 @@ class SequenceEngine:
     def _alloc(self, S, N, device):
         if self._bufs is None or self._bufs[0].shape[0] < S:
             self._graphs.clear()
+            G = self.lstm.G
             self._bufs = tuple(torch.zeros(S, 2, N, dtype=torch.int64, device=device)
-                               for _ in range(3))      # H, C, x
+                               for _ in range(3)) + (      # H, C, x
+                torch.zeros(S, G, 2, N, dtype=torch.int64, device=device),)   # forget-gate LUT
         return self._bufs
+
+    def _step(self, s):
+        H, C, X, L = self._bufs
+        Hn, Cn, Ln = self.lstm.step_pipelined(H[:s], C[:s], X[:s], L[:s])
+        H[:s].copy_(Hn)
+        C[:s].copy_(Cn)
+        L[:s].copy_(Ln)
 
     def _graph_for(self, s):
         if s not in self._graphs:
-            H, C, X = self._bufs
+            H, C, X, L = self._bufs
             # eager warm-up at this size (lazy library state, launch-shape
             # tuning); the step leaves its inputs untouched
             st = torch.cuda.Stream()
             st.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(st):
-                self.lstm.step(H[:s], C[:s], X[:s])
+                self.lstm.step_pipelined(H[:s], C[:s], X[:s], L[:s])
             torch.cuda.current_stream().wait_stream(st)
             if self._pool is None:
                 self._pool = torch.cuda.graph_pool_handle()
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, pool=self._pool):
-                Hn, Cn = self.lstm.step(H[:s], C[:s], X[:s])
-                H[:s].copy_(Hn)
-                C[:s].copy_(Cn)
+                self._step(s)
             self._graphs[s] = g
         return self._graphs[s]
 
@@ -64,9 +74,10 @@ class SequenceEngine:
         order = sorted(range(S), key=lambda i: -lens[i])
         N = int(sequences[0][0].shape[-1])
         dev = sequences[0][0].device
-        H, C, X = self._alloc(S, N, dev)
+        H, C, X, L = self._alloc(S, N, dev)
         H[:S].zero_()
         C[:S].zero_()
+        L[:S].copy_(self.lstm.forget_lut(C[:S]))
         # inputs in length order, padded to the longest: (T_max, S, 2, N)
         T_max = max(lens)
         xs = torch.zeros(T_max, S, 2, N, dtype=torch.int64, device=dev)
@@ -85,9 +96,7 @@ class SequenceEngine:
             if self.use_graph:
                 self._graph_for(live).replay()
             else:
-                Hn, Cn = self.lstm.step(H[:live], C[:live], X[:live])
-                H[:live].copy_(Hn)
-                C[:live].copy_(Cn)
+                self._step(live)
             if step_times is not None:
                 e1.record(stream)
                 step_times.append((live, e0, e1))

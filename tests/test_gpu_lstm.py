@@ -363,6 +363,33 @@ def test_lstm_timesteps_decrypt_to_the_plaintext_rule(cuda_device):
     assert checked >= total // 4, (checked, total)
 
 
+@pytest.mark.parametrize("overlap", [True, False])
+def test_pipelined_steps_equal_plain_steps(cuda_device, overlap):
+    """step_pipelined (the forget gate's LUT prepared one step ahead, on a
+    side stream when overlap is on) gives the states of plain step() calls
+    bit for bit, for 3 sequences over 3 steps."""
+    from schemas.RLWE_jax import encrypt
+    m = _Model(seed=3)
+    lstm, sk = _build_device_layer(m)
+    lstm.OVERLAP_MAX_BOOTSTRAPS = 1 << 20 if overlap else 0
+    rng = np.random.default_rng(9)
+    S, T_ = 3, 3
+    msg = np.zeros((T_ * S, m.N), np.int64)
+    msg[:, :m.input_dim] = rng.integers(0, 2, (T_ * S, m.input_dim)) << 56
+    A, b = encrypt(msg, sk, m.dp, rng)
+    x = torch.stack([dev(A), dev(b)], dim=1).reshape(T_, S, 2, m.N)
+    H = torch.zeros(S, 2, m.N, dtype=torch.int64, device="cuda:0")
+    C = torch.zeros_like(H)
+    H2, C2 = H.clone(), C.clone()
+    lut = lstm.forget_lut(C2)
+    for t in range(T_):
+        H, C = lstm.step(H, C, x[t])
+        H2, C2, lut = lstm.step_pipelined(H2, C2, x[t], lut)
+        np.testing.assert_array_equal(host(H2), host(H), err_msg=f"H, step {t}")
+        np.testing.assert_array_equal(host(C2), host(C), err_msg=f"C, step {t}")
+        np.testing.assert_array_equal(host(lut), host(lstm.forget_lut(C)), err_msg=f"LUT, step {t}")
+
+
 def test_engine_uneven_lengths_and_graph_replays_match_eager(cuda_device):
     """Sequences of lengths 3, 1, 2 through the timestep engine: each final
     state equals the state of that sequence run alone (eager), and replays of
