@@ -209,11 +209,12 @@ __global__ __launch_bounds__(64 * KM_WAVES) void ks_mfma_kernel(
     }
 }
 
-// Larger batches with one key: a workgroup covers one 32-column tile and
-// KM4_MT = 4 row tiles; wave w takes row tiles 2*(w/2), 2*(w/2)+1 and limbs
-// 4*(w%2) .. 4*(w%2)+3, so each wave loads 2 digit tiles and 4 key tiles per
-// k step for 8 MFMAs (the kernel above loads 9 for 8) and a key tile serves
-// four row tiles.  The two limb halves of an output meet in LDS before the
+// Batches of two or more row tiles with one key: a workgroup covers one
+// 32-column tile and KM4_MT = 4 row tiles; wave w takes row tiles 2*(w/2),
+// 2*(w/2)+1 and limbs 4*(w%2) .. 4*(w%2)+3, so each wave loads 2 digit tiles
+// and 4 key tiles per k step for 8 MFMAs (the kernel above loads 9 for 8) and
+// a key tile serves four row tiles.  Waves past the last row tile only join
+// the barrier.  The two limb halves of an output meet in LDS before the
 // atomic add.
 constexpr int KM4_MT = 4;
 
@@ -364,8 +365,9 @@ static int ks_mfma_run(void *stream, const int64_t *a_in, const int64_t *b_in,
     const unsigned eb = (unsigned)std::min<int64_t>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(ks_out_init_kernel, dim3(eb), dim3(256), 0, s, b_in, (int)batch, (int)n_out,
                        a_out, b_out);
-    if (!tile_slots && MT >= KM4_MT) {
-        // one key, at least four row tiles: four row tiles per workgroup;
+    if (!tile_slots && MT >= 2) {
+        // one key, two or more row tiles: four row tiles per workgroup (the
+        // key is then read once per four row tiles, not once per tile);
         // split K until the grid has ~4 workgroups per CU
         const int gy = (MT + KM4_MT - 1) / KM4_MT;
         int ks = (int)std::max<int64_t>(1, std::min<int64_t>(KT, 1024 / ((int64_t)NT * gy)));

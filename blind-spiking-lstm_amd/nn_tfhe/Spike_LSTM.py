@@ -170,7 +170,8 @@ class CipherSpikeLSTM:
         That LUT depends only on the new cell state, so it is prepared while
         the output gate's bootstrap runs -- on a side stream when that
         bootstrap is small -- and leaves the sequential path of the next
-        step.  Results are those of step(), bit for bit."""
+        step.  The three gates share one key switch (exact, so batching it
+        changes nothing).  Results are those of step(), bit for bit."""
         S, N = H.shape[0], self.degree
         G, nl, lq = self.G, self.n_lut, self.log_q
         # HX = x_t + H; the three gate pre-activations
@@ -178,14 +179,16 @@ class CipherSpikeLSTM:
         w, bias, sup = self._gate_weights()
         gm, gb = torus.linear_extract(HX, w, bias, lq, sup, check_support=False)
         Gf, Gi = self.W_f.kernel_polynomial.shape[0], self.W_i.kernel_polynomial.shape[0]
-        split = (Gf, Gi, gm.shape[1] - Gf - Gi)
-        F_t, I_t, O_t = ((m_, b_.unsqueeze(-1)) for m_, b_ in zip(gm.split(split, dim=1),
-                                                                  gb.split(split, dim=1)))
+        heavy = self.heavyBS
+        # one key switch for the three gates (rows f, i, o of every sequence)
+        ka, kb = heavy.key_switch((gm.reshape(-1, N), gb.reshape(-1, 1)))
+        ka, kb = ka.reshape(S, -1, ka.shape[-1]), kb.reshape(S, -1, 1)
+        fi_ks = (ka[:, :Gf + Gi].reshape(-1, ka.shape[-1]), kb[:, :Gf + Gi].reshape(-1, 1))
+        o_ks = (ka[:, Gf + Gi:].reshape(-1, ka.shape[-1]), kb[:, Gf + Gi:].reshape(-1, 1))
         # first bootstrap: forget gate with the C-dependent LUT, input gate
-        inp = (torch.cat([F_t[0], I_t[0]], dim=1), torch.cat([F_t[1], I_t[1]], dim=1))
         lut = torch.cat([lut_C, self.lut_i.unsqueeze(0).expand(S, G, 2, N)], dim=1)
-        bm, bb = self.heavyBS.cuboot((inp[0].reshape(S * 2 * G, N), inp[1].reshape(-1, 1)),
-                                     [lut[:, :, 0].reshape(-1, N), lut[:, :, 1].reshape(-1, N)])
+        bm, bb = heavy.cuboot_switched(fi_ks, [lut[:, :, 0].reshape(-1, N),
+                                               lut[:, :, 1].reshape(-1, N)])
         bm = bm.reshape(S, 2 * G, nl, N)
         bb = bb.reshape(S, 2 * G, nl)
         sI = self._spikes(bm, bb, G, 2 * G)
@@ -209,8 +212,8 @@ class CipherSpikeLSTM:
         # second bootstrap: output gate with the C_t-dependent LUT
         lut_H = self.heavyBS.prepare_LUT_sum((C_t[0].reshape(S, nl, G, N),
                                               C_t[1].reshape(S, nl, G)), self.lut_o_fft)
-        hm, hb = self.heavyBS.cuboot((O_t[0].reshape(S * G, N), O_t[1].reshape(-1, 1)),
-                                     [lut_H[:, :, 0].reshape(-1, N), lut_H[:, :, 1].reshape(-1, N)])
+        hm, hb = heavy.cuboot_switched(o_ks, [lut_H[:, :, 0].reshape(-1, N),
+                                              lut_H[:, :, 1].reshape(-1, N)])
         H_t = self._spikes(hm.reshape(S, G, nl, N), hb.reshape(S, G, nl), 0, G)
         if side is None:
             C_new, lut_next = pack_c()
@@ -221,8 +224,8 @@ class CipherSpikeLSTM:
             main.wait_stream(side)
             if not torch.cuda.is_current_stream_capturing():
                 # made on the side stream, used on this one from here on
-                C_new.record_stream(main)
-                lut_next.record_stream(main)
+                for t_ in (C_new, lut_next):
+                    t_.record_stream(main)
         return H_new, C_new, lut_next
 
     def __call__(self, x, seq_len=None):

@@ -103,7 +103,15 @@ class HeavysideBoostrapper:
                              p["l_bs"], p["degree"], self.collapse, self.all_rot_possible_fourier)
 
     def cuboot(self, h, encrypted_LUT):
-        c_lwe_ks = key_switch_LWE(self.key_switching_key_bs, h, self.dict_params_ks_LWE)
+        return self.cuboot_switched(self.key_switch(h), encrypted_LUT)
+
+    def key_switch(self, h):
+        """The key switch cuboot starts with (activation.py:150)."""
+        return key_switch_LWE(self.key_switching_key_bs, h, self.dict_params_ks_LWE)
+
+    def cuboot_switched(self, c_lwe_ks, encrypted_LUT):
+        """cuboot after its key switch (activation.py:151-162): blind
+        rotation of the LUT and many-LUT extraction."""
         p = self.dict_params_lut
         return cuboot_merge(c_lwe_ks, encrypted_LUT, self.bootstrapping_key, p["q"], p["beta_bs"],
                             p["l_bs"], p["degree"], self.collapse, self.all_rot_possible_fourier,

@@ -118,11 +118,13 @@ class SlotBootstrapper:
         return torus.lut_product_sum(rl, lut_fft, masks.shape[-3], log2_int(p["q"]))
 
     def cuboot(self, h, encrypted_LUT):
-        """Key switch + many-LUT bootstrap of rows ordered by sequence.  Each
-        sequence's rows are padded to a multiple of 32 for the key switch so
-        that no 32-row tile mixes clients; bootstrap pairs share a key when a
-        sequence has an even number of rows (else only the forms with one
-        ciphertext per workgroup are tuned over)."""
+        """Key switch + many-LUT bootstrap of rows ordered by sequence."""
+        return self.cuboot_switched(self.key_switch(h), encrypted_LUT)
+
+    def key_switch(self, h):
+        """Key switch of rows ordered by sequence, each sequence's with its
+        client's key.  Each sequence's rows are padded to a multiple of 32
+        so that no 32-row tile mixes clients.  Returns (a (rows, n), b (rows, 1))."""
         pk, pb = h
         rows, n_in = pk.shape[0], pk.shape[-1]
         S = self.seq_slots.numel()
@@ -141,13 +143,23 @@ class SlotBootstrapper:
         if pad:
             a_ks = a_ks.reshape(S, per_seq + pad, -1)[:, :per_seq].reshape(rows, -1)
             b_ks = b_ks.reshape(S, per_seq + pad)[:, :per_seq].reshape(rows)
+        return a_ks, b_ks.reshape(rows, 1)
+
+    def cuboot_switched(self, c_lwe_ks, encrypted_LUT):
+        """cuboot after its key switch: many-LUT bootstrap of rows ordered by
+        sequence, each with its client's bootstrapping key.  Bootstrap pairs
+        share a key when a sequence has an even number of rows (else only the
+        forms with one ciphertext per workgroup are tuned over)."""
+        a_ks, b_ks = c_lwe_ks
+        rows = a_ks.shape[0]
+        per_seq = rows // self.seq_slots.numel()
         p = self.dict_params_lut
         idx = torus.rotation_indices(a_ks, log2_int(p["q"]), p["degree"], self.collapse, self.n_lut)
         lut = torch.stack([torus._i64(encrypted_LUT[0]), torus._i64(encrypted_LUT[1])], dim=1)
         br = mul_bootstrap(lut, self.pool.pools["bsk"], None, idx, log2_int(p["beta_bs"]),
                            log2_int(p["q"]), p["l_bs"], form="auto", slots=self._per(rows),
                            candidates=FORMS if per_seq % 2 == 0 else ("plain", "rows"))
-        masks, bodies = torus.rotate_extract(br, b_ks, log2_int(p["q"]), self.n_lut)
+        masks, bodies = torus.rotate_extract(br, b_ks.reshape(rows), log2_int(p["q"]), self.n_lut)
         return masks, bodies[..., None]
 
     def packing(self, x):

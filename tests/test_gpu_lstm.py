@@ -113,9 +113,9 @@ def test_key_switch_i64_bit_exact(cuda_device, T, log_q, lb, l, n_in, n_out, B):
                                                      (32, 4, 5, 1024, 630, 129)])
 def test_key_switch_mfma_bit_exact(cuda_device, T, log_q, lb, l, n_in, n_out, B):
     """The int8 limb MFMA key switch equals the exact oracle (and so the
-    VALU int64 key switch) bit for bit, including ragged tiles; B >= 97
-    (four or more row tiles) runs the four-row-tile kernel, B = 150 with a
-    lone last row tile."""
+    VALU int64 key switch) bit for bit, including ragged tiles; B >= 33
+    (two or more row tiles) runs the four-row-tile kernel, B = 40 and 70
+    with idle waves, B = 150 with a lone last row tile."""
     rng = np.random.default_rng(l * n_in + B)
     s_old, s_new = rng.integers(0, 2, n_in), rng.integers(0, 2, n_out)
     ka, kb = L.keyswitch_key(rng, s_new, s_old, log_q, lb, l, 2.0 ** (log_q - 20))
