@@ -183,6 +183,7 @@ class Request:
     slot: int = -1
     H: torch.Tensor | None = None
     C: torch.Tensor | None = None
+    L: torch.Tensor | None = None    # forget-gate LUT of the next timestep (G, 2, N)
     start: float = -1.0
     done: float = -1.0
     result: tuple = field(default_factory=tuple)
@@ -243,16 +244,27 @@ class EncryptedLSTMServer:
         H = torch.stack([r.H for r in act])
         C = torch.stack([r.C for r in act])
         X = torch.stack([r.x[r.t] for r in act])
+        new = [i for i, r in enumerate(act) if r.L is None]
+        if new:
+            # first timestep of these requests: the forget-gate LUT of their
+            # (zero) cell state, under each client's keys
+            self.bs.set_slots(torch.as_tensor([act[i].slot for i in new], dtype=torch.int32))
+            Ln = self.lstm.forget_lut(C[new])
+            for j, i in enumerate(new):
+                act[i].L = Ln[j]
+        L = torch.stack([r.L for r in act])
         self.bs.set_slots(torch.as_tensor([r.slot for r in act], dtype=torch.int32))
-        Hn, Cn = self.lstm.step(H, C, X)
+        # the step also prepares each request's next forget-gate LUT
+        Hn, Cn, Ln = self.lstm.step_pipelined(H, C, X, L)
         self.ticks += 1
         self.bootstraps += len(act) * 3 * self.lstm.G
         keep = []
         for i, r in enumerate(act):
-            r.H, r.C = Hn[i], Cn[i]
+            r.H, r.C, r.L = Hn[i], Cn[i], Ln[i]
             r.t += 1
             if r.t == r.x.shape[0]:
                 r.result = (r.H, r.C)
+                r.L = None
                 self.pool.release(r.slot)
                 self.finished.append(r)
             else:
