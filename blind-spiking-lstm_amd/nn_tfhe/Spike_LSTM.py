@@ -164,9 +164,10 @@ class CipherSpikeLSTM:
         return H_new, C_new
 
     def step_pipelined(self, H, C, x_t, lut_C, next_lut: bool = True):
-        """step() with the forget gate's LUT given (forget_lut(C), or the
-        third result of the previous step_pipelined call) and, if next_lut,
-        the next step's LUT returned as well: (H_t, C_t, forget_lut(C_t)).
+        """The timestep of Spike_LSTM.py:108-178 as step(), with the forget
+        gate's LUT given (forget_lut(C), or the third result of the previous
+        step_pipelined call) and, if next_lut, the next step's LUT returned as
+        well: (H_t, C_t, forget_lut(C_t)).
         That LUT depends only on the new cell state, so it is prepared while
         the output gate's bootstrap runs -- on a side stream when that
         bootstrap is small -- and leaves the sequential path of the next
@@ -196,7 +197,7 @@ class CipherSpikeLSTM:
         C_t = self._lwe_add(sFC, sI)                                            # (S, hidden, .)
 
         def pack_c():
-            Cp = self.heavyBS.packing((C_t[0], C_t[1].unsqueeze(-1)))
+            Cp = heavy.packing((C_t[0], C_t[1].unsqueeze(-1)))
             C_new = torch.stack([Cp[0], Cp[1]], dim=1)
             return C_new, (self.forget_lut(C_new) if next_lut else None)
 
@@ -210,15 +211,15 @@ class CipherSpikeLSTM:
             with torch.cuda.stream(side):
                 C_new, lut_next = pack_c()
         # second bootstrap: output gate with the C_t-dependent LUT
-        lut_H = self.heavyBS.prepare_LUT_sum((C_t[0].reshape(S, nl, G, N),
-                                              C_t[1].reshape(S, nl, G)), self.lut_o_fft)
+        lut_H = heavy.prepare_LUT_sum((C_t[0].reshape(S, nl, G, N),
+                                       C_t[1].reshape(S, nl, G)), self.lut_o_fft)
         hm, hb = heavy.cuboot_switched(o_ks, [lut_H[:, :, 0].reshape(-1, N),
                                               lut_H[:, :, 1].reshape(-1, N)])
         H_t = self._spikes(hm.reshape(S, G, nl, N), hb.reshape(S, G, nl), 0, G)
         if side is None:
             C_new, lut_next = pack_c()
         # pack H back into an RLWE; it moves behind the input slots
-        Hp = self.heavyBS.packing((H_t[0], H_t[1].unsqueeze(-1)))
+        Hp = heavy.packing((H_t[0], H_t[1].unsqueeze(-1)))
         H_new = torus.rotate_poly(torch.stack([Hp[0], Hp[1]], dim=1), self._shift, lq)
         if side is not None:
             main.wait_stream(side)
