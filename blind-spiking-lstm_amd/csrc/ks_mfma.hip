@@ -156,7 +156,7 @@ __global__ void ks_out_final_kernel(int batch, int n_out, int lq, int64_t *__res
 
 // tile_slots (optional): the key of row tile mt is ktiles + tile_slots[mt] *
 // slot_bytes (clamped to [0, slot_count)), for batches that mix clients
-__global__ __launch_bounds__(64 * KM_WAVES) void ks_mfma_kernel(
+__global__ __launch_bounds__(64 * KM_WAVES, 2) void ks_mfma_kernel(
     const int8_t *__restrict__ dtiles, const int8_t *__restrict__ ktiles, int batch, int n_out,
     int KT, int NT, int kt_per_split, const int32_t *__restrict__ tile_slots, int64_t slot_bytes,
     int slot_count, int64_t *__restrict__ a_out, int64_t *__restrict__ b_out)
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(64 * KM_WAVES) void ks_mfma_kernel(
 // atomic add.
 constexpr int KM4_MT = 4;
 
-__global__ __launch_bounds__(256) void ks_mfma4_kernel(const int8_t *__restrict__ dtiles,
+__global__ __launch_bounds__(256, 2) void ks_mfma4_kernel(const int8_t *__restrict__ dtiles,
                                                        const int8_t *__restrict__ ktiles, int batch,
                                                        int n_out, int KT, int NT, int MT,
                                                        int kt_per_split, int64_t *__restrict__ a_out,
@@ -257,24 +257,21 @@ __global__ __launch_bounds__(256) void ks_mfma4_kernel(const int8_t *__restrict_
             }
         }
     }
-    // this wave's limbs combined: sum_j acc_j << 8*(4*lh + j) mod 2^64
-    uint64_t part[2][16];
+    // this wave's limbs combined, sum_j acc_j << 8*(4*lh + j) mod 2^64, one
+    // output at a time (no array of partial sums held beside the accumulators)
+    auto limbs = [&](int t, int r) -> uint64_t {
+        uint64_t v = 0;
 #pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            uint64_t v = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                v += (uint64_t)(int64_t)acc[t][j][r] << (8 * (4 * lh + j));
-            part[t][r] = v;
-        }
+        for (int j = 0; j < 4; j++)
+            v += (uint64_t)(int64_t)acc[t][j][r] << (8 * (4 * lh + j));
+        return v;
+    };
     if (lh == 1)
 #pragma unroll
         for (int t = 0; t < 2; t++)
 #pragma unroll
             for (int r = 0; r < 16; r++)
-                upper[mp][t][r][lane] = part[t][r];
+                upper[mp][t][r][lane] = limbs(t, r);
     __syncthreads();
     const int n = nt * 32 + (lane & 31);
     if (lh != 0 || n > n_out)
@@ -291,7 +288,7 @@ __global__ __launch_bounds__(256) void ks_mfma4_kernel(const int8_t *__restrict_
                 continue;
             unsigned long long *dst = reinterpret_cast<unsigned long long *>(
                 n < n_out ? a_out + (size_t)m * n_out + n : b_out + m);
-            atomicAdd(dst, (unsigned long long)(part[t][r] + upper[mp][t][r][lane]));
+            atomicAdd(dst, (unsigned long long)(limbs(t, r) + upper[mp][t][r][lane]));
         }
     }
 }
