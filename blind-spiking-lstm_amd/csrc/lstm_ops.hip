@@ -175,7 +175,7 @@ __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
 // step runs on twice the threads (measured 9-12 % faster than one group from
 // 128 to 4096 LWEs at N = 2048, l = 20).  RG = 1: one group (other N).
 template <int N, int RG>
-__global__ __launch_bounds__(N / 8 * RG) void trace_pack_kernel(
+__global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void trace_pack_kernel(
     const int64_t *__restrict__ lwe_a, const int64_t *__restrict__ lwe_b,
     const cplx *__restrict__ gkey, int l, int lb, int lq, int64_t *__restrict__ out,
     const int32_t *__restrict__ slots, int slot_count)
@@ -183,6 +183,8 @@ __global__ __launch_bounds__(N / 8 * RG) void trace_pack_kernel(
     constexpr int M = Fold<N>::M, T = Fold<N>::T, LOGN = __builtin_ctz(N);
     static_assert(RG == 1 || (RG == 2 && Wave<N>::OK), "two groups need the wave-local transform");
     constexpr int NR = 2 / RG;               // rows (A, B) a group keeps in registers
+    // one group where a two-group form exists: same arithmetic as that form
+    constexpr bool DUAL = RG == 1 && Wave<N>::OK;
     __shared__ cplx buf[RG * 2 * M];         // per group: two transform polynomials
     __shared__ cplx ztab[N];
     // 2N int64 for the permutation: aliases the last group's buffer, whose
@@ -275,16 +277,11 @@ __global__ __launch_bounds__(N / 8 * RG) void trace_pack_kernel(
         // transforms the level pairs h, h + RG, ..., starting the digit
         // recursion at each pair from residual_at
         const cplx *key = gkey + (size_t)st * l * 2 * M;
-        cplx acc[2][2][2];
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int pp = 0; pp < 2; pp++)
-                acc[a][pp][0] = acc[a][pp][1] = {0.0, 0.0};
-#pragma unroll 1
-        for (int it = 0; it < iters; it++) {
+        // digits of level pair P, their transforms, and the products with the
+        // key rows of those levels added into A (in level order)
+        auto pair_mac = [&](int P, cplx (&A)[2][2][2]) {
             cplx v[2][4];
-            const int lev = 2 * (it * RG + h);
+            const int lev = 2 * P;
             double dg[2][8];
 #pragma unroll
             for (int i = 0; i < 8; i++) {
@@ -311,11 +308,35 @@ __global__ __launch_bounds__(N / 8 * RG) void trace_pack_kernel(
 #pragma unroll
                     for (int a = 0; a < 2; a++) {
                         const cplx *kb = key + ((size_t)(lev + c) * 2 + a) * M;
-                        acc[a][pp][0] = cfma(d0, kb[2 * rb], acc[a][pp][0]);
-                        acc[a][pp][1] = cfma_conj(kb[2 * rb + 1], d1, acc[a][pp][1]);
+                        A[a][pp][0] = cfma(d0, kb[2 * rb], A[a][pp][0]);
+                        A[a][pp][1] = cfma_conj(kb[2 * rb + 1], d1, A[a][pp][1]);
                     }
                 }
             }
+        };
+        // acc sums the pairs h, h + 2, ... (two groups) or all pairs (one
+        // group); with DUAL one group keeps the even and odd pairs' sums apart
+        // (acc, acc1) and adds them at the end, which is the two-group form's
+        // arithmetic exactly
+        cplx acc[2][2][2], acc1[2][2][2];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int pp = 0; pp < 2; pp++) {
+                acc[a][pp][0] = acc[a][pp][1] = {0.0, 0.0};
+                acc1[a][pp][0] = acc1[a][pp][1] = {0.0, 0.0};
+            }
+        if constexpr (DUAL) {
+#pragma unroll 1
+            for (int it = 0; it < (pairs + 1) / 2; it++) {
+                pair_mac(2 * it, acc);
+                if (2 * it + 1 < pairs)
+                    pair_mac(2 * it + 1, acc1);
+            }
+        } else {
+#pragma unroll 1
+            for (int it = 0; it < iters; it++)
+                pair_mac(it * RG + h, acc);
         }
         __syncthreads();
         if constexpr (RG == 1) {
@@ -324,8 +345,9 @@ __global__ __launch_bounds__(N / 8 * RG) void trace_pack_kernel(
                 const int rb = t + pp * T;
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
-                    buf[a * M + phys<N>(rb)] = acc[a][pp][0];
-                    buf[a * M + phys<N>(M - 1 - rb)] = acc[a][pp][1];
+                    buf[a * M + phys<N>(rb)] = DUAL ? cadd(acc[a][pp][0], acc1[a][pp][0]) : acc[a][pp][0];
+                    buf[a * M + phys<N>(M - 1 - rb)] =
+                        DUAL ? cadd(acc[a][pp][1], acc1[a][pp][1]) : acc[a][pp][1];
                 }
             }
         } else {
@@ -815,7 +837,10 @@ static int pack_lwe_run(void *stream, const int64_t *lwe_a, const int64_t *lwe_b
     if (!lwe_a || !lwe_b || !galois_key_fft || !out)
         return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
     const cplx *gk = reinterpret_cast<const cplx *>(galois_key_fft);
-    const bool two = degree >= 512 && degree <= 2048;
+    // two thread groups per LWE for latency (fewer LWEs than two per CU);
+    // one group, two workgroups per CU, for throughput -- both forms give the
+    // same result for every LWE, so it does not depend on the batch
+    const bool two = degree >= 512 && degree <= 2048 && count < 512;
     if (two) {
         BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((trace_pack_kernel<NN, Wave<NN>::OK ? 2 : 1>),
                                                         dim3((unsigned)count),

@@ -185,6 +185,22 @@ def test_trace_pack_vs_exact(cuda_device, T, N, log_q, lb, l, sigma, cnt):
         assert max_err(ph_got, ph_want, log_q) <= TOL_REL * 2.0 ** log_q
 
 
+@pytest.mark.parametrize("N,log_q,lb,l", [(2048, 64, 2, 20), (1024, 64, 2, 20), (512, 32, 4, 5)])
+def test_trace_pack_same_in_any_batch(cuda_device, T, N, log_q, lb, l):
+    """Trace packing runs one thread group per LWE for 512 or more LWEs and
+    two for fewer; an LWE's result must not depend on the batch: 640 LWEs
+    in one call equal the same LWEs in calls of 128, bit for bit (l = 5
+    has an odd number of level pairs)."""
+    rng = np.random.default_rng(N * l)
+    S = rng.integers(0, 2, N)
+    gk = _gkey_fft(T, L.galois_key(rng, S, log_q, lb, l, 2.0 ** (log_q - 30)))
+    a, b = dev(L.uniform(rng, (640, N), log_q)), dev(L.uniform(rng, 640, log_q))
+    whole = host(T.pack_lwe_to_rlwe(a, b, gk, lb, l, log_q))
+    parts = np.concatenate([host(T.pack_lwe_to_rlwe(a[i:i + 128], b[i:i + 128], gk, lb, l, log_q))
+                            for i in range(0, 640, 128)])
+    np.testing.assert_array_equal(whole, parts)
+
+
 @pytest.mark.parametrize("N,log_q,S_,G,nl", [(2048, 64, 2, 8, 4), (256, 64, 1, 3, 4), (512, 32, 3, 2, 2)])
 def test_lut_product_sum_vs_exact(cuda_device, T, N, log_q, S_, G, nl):
     rng = np.random.default_rng(N * G)
