@@ -49,6 +49,15 @@ __device__ __forceinline__ void inverse_any(cplx (&v)[2][4], cplx *buf, const cp
         inverse_pair<N>(v, buf, ztab, t);
 }
 
+template <int N>
+__device__ __forceinline__ void forward_one(cplx (&v)[1][4], cplx *buf, const cplx *ztab, int t)
+{
+    if constexpr (Wave<N>::OK)
+        forward_wave<N, false, 1>(v, buf, ztab, t);
+    else
+        forward_pair<N, false, 1>(v, buf, ztab, t);
+}
+
 // fold + twist of two real polynomials given by their coefficients
 // lo[r] = p_{t+rT}, hi[r] = p_{t+rT+N/2}: u_k = (p_k - i p_{k+N/2}) zeta^k
 template <int N>
@@ -570,7 +579,7 @@ __global__ __launch_bounds__(N / 8) void packing_mac_kernel(const int64_t *__res
                                                             int slot_count)
 {
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
-    __shared__ cplx buf[2 * M];
+    __shared__ cplx buf[M];   // one transform at a time: 48 KiB of LDS, three workgroups per CU
     __shared__ cplx ztab[N];
     const int t = threadIdx.x;
     const int p = blockIdx.y, blk = blockIdx.x, NB = gridDim.x;
@@ -597,35 +606,26 @@ __global__ __launch_bounds__(N / 8) void packing_mac_kernel(const int64_t *__res
         for (int i = 0; i < 8; i++)
             r[i] = t + i * T < n_lwes ? row[t + i * T] : 0;
 #pragma unroll 1
-        for (int lev = 0; lev < l; lev += 2) {
-            cplx v[2][4];
+        for (int lev = 0; lev < l; lev++) {
+            cplx v[1][4];
+            double dg[8];
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
-                const int lv = lev + c;
-                double dg[8];
+            for (int i = 0; i < 8; i++)
+                dg[i] = (double)next_digit(r[i], lq - (lev + 1) * lb);
 #pragma unroll
-                for (int i = 0; i < 8; i++)
-                    dg[i] = lv < l ? (double)next_digit(r[i], lq - (lv + 1) * lb) : 0.0;
-#pragma unroll
-                for (int rr = 0; rr < 4; rr++)
-                    v[c][rr] = fold_point<N>(dg[rr], dg[rr + 4], t, rr, ztab);
-            }
-            forward_any<N>(v, buf, ztab, t);
+            for (int rr = 0; rr < 4; rr++)
+                v[0][rr] = fold_point<N>(dg[rr], dg[rr + 4], t, rr, ztab);
+            forward_one<N>(v, buf, ztab, t);
 #pragma unroll
             for (int pp = 0; pp < 2; pp++) {
                 const int rb = t + pp * T;
+                const cplx d0 = buf[phys<N>(rb)];
+                const cplx d1 = buf[phys<N>(M - 1 - rb)];
 #pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    if (lev + c >= l)
-                        break;
-                    const cplx d0 = buf[c * M + phys<N>(rb)];
-                    const cplx d1 = buf[c * M + phys<N>(M - 1 - rb)];
-#pragma unroll
-                    for (int a = 0; a < 2; a++) {
-                        const cplx *kb = ksk + (((size_t)j * l + lev + c) * 2 + a) * M;
-                        acc[a][pp][0] = cfma(d0, kb[2 * rb], acc[a][pp][0]);
-                        acc[a][pp][1] = cfma_conj(kb[2 * rb + 1], d1, acc[a][pp][1]);
-                    }
+                for (int a = 0; a < 2; a++) {
+                    const cplx *kb = ksk + (((size_t)j * l + lev) * 2 + a) * M;
+                    acc[a][pp][0] = cfma(d0, kb[2 * rb], acc[a][pp][0]);
+                    acc[a][pp][1] = cfma_conj(kb[2 * rb + 1], d1, acc[a][pp][1]);
                 }
             }
         }
