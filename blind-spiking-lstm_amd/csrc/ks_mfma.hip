@@ -26,6 +26,7 @@
 // alone gives too few; partial int64 results meet in the outputs through
 // exact 64-bit integer atomics (order-independent, bit-reproducible).
 #include <algorithm>
+#include <mutex>
 
 #include "capi_internal.h"
 #include "torus_device.h"
@@ -413,4 +414,91 @@ extern "C" int bslstm_key_switch_lwe_mfma_slots(void *stream, const int64_t *a_i
         return fail(BSLSTM_INVALID_ARGUMENT, "tile_slots and 1 <= slot_count required");
     return ks_mfma_run(stream, a_in, b_in, key_pool, tile_slots, slot_count, batch, n_in, n_out,
                        log2_beta_ks, l_ks, log2_q, workspace, workspace_bytes, a_out, b_out);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form.  The digit tiles (the only device memory the key
+// switch needs beyond its operands) live in a file-scope device array of
+// KS_NOALLOC_SCRATCH_BYTES; every GPU holds its own copy of it until the
+// process ends.  Host code never passes the array's name to a kernel: its
+// device address is looked up once per device with hipGetSymbolAddress (that
+// device current) and kept.  All calls on one device share the array, so
+// calls that overlap on one device (two streams, or two host threads) race;
+// the digit kernel rewrites every tile a call reads, so nothing is cleared.
+// ---------------------------------------------------------------------------
+namespace bslstm {
+
+constexpr int64_t KS_NOALLOC_SCRATCH_BYTES = 16LL << 20;
+constexpr int KS_NOALLOC_MAX_DEVICES = 64;
+
+__device__ __align__(256) int8_t scratch[KS_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[KS_NOALLOC_MAX_DEVICES];
+
+// Device address of `scratch` on the device that runs `stream`.
+static int scratch_address(hipStream_t stream, void **out)
+{
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess)
+        return fail(BSLSTM_INTERNAL, "hipGetDevice: %s", hipGetErrorString(e));
+    int dev = cur;
+    if (stream) {
+        hipDevice_t sd = 0;
+        e = hipStreamGetDevice(stream, &sd);
+        if (e != hipSuccess)
+            return fail(BSLSTM_INVALID_ARGUMENT, "hipStreamGetDevice: %s", hipGetErrorString(e));
+        dev = (int)sd;
+    }
+    if (dev < 0 || dev >= KS_NOALLOC_MAX_DEVICES)
+        return fail(BSLSTM_INVALID_ARGUMENT, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        if (dev != cur && (e = hipSetDevice(dev)) != hipSuccess)
+            return fail(BSLSTM_INTERNAL, "hipSetDevice(%d): %s", dev, hipGetErrorString(e));
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (dev != cur)
+            (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(BSLSTM_INTERNAL, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return BSLSTM_OK;
+}
+
+}  // namespace bslstm
+
+extern "C" int64_t bslstm_key_switch_lwe_mfma_noalloc_max_bytes(void)
+{
+    return bslstm::KS_NOALLOC_SCRATCH_BYTES;
+}
+
+extern "C" int bslstm_key_switch_lwe_mfma_noalloc(void *stream, const int64_t *a_in,
+                                                  const int64_t *b_in, const int8_t *key_tiles,
+                                                  int64_t batch, int64_t n_in, int64_t n_out,
+                                                  int64_t log2_beta_ks, int64_t l_ks,
+                                                  int64_t log2_q, int64_t *a_out, int64_t *b_out)
+{
+    clear_error();
+    const int64_t need = bslstm_ks_mfma_workspace_bytes(batch, l_ks, n_in);
+    if (need < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad shapes (batch >= 0, l_ks >= 1, n_in >= 1)");
+    if (need > KS_NOALLOC_SCRATCH_BYTES)
+        return fail(BSLSTM_INVALID_ARGUMENT,
+                    "batch %lld x n_in %lld x l_ks %lld needs %lld bytes of digit tiles; the "
+                    "allocation-free key switch holds at most %lld",
+                    (long long)batch, (long long)n_in, (long long)l_ks, (long long)need,
+                    (long long)KS_NOALLOC_SCRATCH_BYTES);
+    if (batch == 0)
+        return ks_mfma_run(stream, a_in, b_in, key_tiles, nullptr, 1, batch, n_in, n_out,
+                           log2_beta_ks, l_ks, log2_q, nullptr, 0, a_out, b_out);
+    void *ws = nullptr;
+    if (int st = scratch_address((hipStream_t)stream, &ws))
+        return st;
+    return ks_mfma_run(stream, a_in, b_in, key_tiles, nullptr, 1, batch, n_in, n_out, log2_beta_ks,
+                       l_ks, log2_q, ws, KS_NOALLOC_SCRATCH_BYTES, a_out, b_out);
 }

@@ -59,6 +59,9 @@ SIGNATURES = {
     "bslstm_ks_mfma_workspace_bytes": ([_I64, _I64, _I64], ctypes.c_int64),
     "bslstm_key_switch_lwe_mfma": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _I64,
                                     _P, _P], ctypes.c_int),
+    "bslstm_key_switch_lwe_mfma_noalloc_max_bytes": ([], ctypes.c_int64),
+    "bslstm_key_switch_lwe_mfma_noalloc": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64,
+                                            _P, _P], ctypes.c_int),
     "bslstm_mul_bootstrap_slots": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _I64, _P, _I64, _I64, _I64,
                                     _I64, _I64, _I64], ctypes.c_int),
     "bslstm_key_switch_lwe_mfma_slots": ([_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64,

@@ -278,6 +278,25 @@ def key_switch_mfma(key_tiles: torch.Tensor, a: torch.Tensor, b: torch.Tensor, n
     return a_out, b_out
 
 
+def key_switch_mfma_noalloc(key_tiles: torch.Tensor, a: torch.Tensor, b: torch.Tensor,
+                            n_out: int, log_beta: int, l: int, log_q: int, a_out: torch.Tensor,
+                            b_out: torch.Tensor) -> None:
+    """key_switch_mfma into caller-owned outputs a_out (B, n_out), b_out (B,)
+    through the allocation-free entry point (digit tiles in the library's
+    static per-device scratch; nothing is allocated by this call)."""
+    _core.require_device(key_tiles, a, b, a_out, b_out)
+    for t in (a, b, a_out, b_out):
+        if t.dtype != torch.int64 or not t.is_contiguous():
+            raise _core.TFHEInvalidArgument("contiguous int64 operands expected")
+    n_in = a.shape[-1]
+    batch = a.numel() // n_in
+    if b.numel() != batch or a_out.numel() != batch * n_out or b_out.numel() != batch:
+        raise _core.TFHEInvalidArgument("operand sizes do not match the batch")
+    _core.check(_lib().bslstm_key_switch_lwe_mfma_noalloc(
+        _core.stream_of(a), a.data_ptr(), b.data_ptr(), key_tiles.data_ptr(), batch, n_in, n_out,
+        log_beta, l, log_q, a_out.data_ptr(), b_out.data_ptr()))
+
+
 # ---------------------------------------------------------------------------
 # key-slot forms: one launch over a batch that mixes clients, each item
 # reading its client's key from a pool of equally shaped keys

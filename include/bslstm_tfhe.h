@@ -300,6 +300,25 @@ int bslstm_key_switch_lwe_mfma(void *stream, const int64_t *a_in, const int64_t 
                                void *workspace, int64_t workspace_bytes,
                                int64_t *a_out, int64_t *b_out);
 
+/*
+ * Allocation-free form of bslstm_key_switch_lwe_mfma.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same operands, results (bit-identical) and errors as
+ * bslstm_key_switch_lwe_mfma minus the workspace: the int8 digit tiles live
+ * in a static per-device array of bslstm_key_switch_lwe_mfma_noalloc_max_bytes()
+ * bytes (16 MiB), so batches whose bslstm_ks_mfma_workspace_bytes() exceed it
+ * are rejected with BSLSTM_INVALID_ARGUMENT.  The call allocates nothing.
+ * All calls on one device share that array: calls that overlap on one
+ * device (concurrent streams or host threads) race.
+ */
+int64_t bslstm_key_switch_lwe_mfma_noalloc_max_bytes(void);
+
+int bslstm_key_switch_lwe_mfma_noalloc(void *stream, const int64_t *a_in, const int64_t *b_in,
+                                       const int8_t *key_tiles, int64_t batch, int64_t n_in,
+                                       int64_t n_out, int64_t log2_beta_ks, int64_t l_ks,
+                                       int64_t log2_q, int64_t *a_out, int64_t *b_out);
+
 /* ------------------------------------------------------------------------
  * Key-slot forms: one launch over a batch that mixes clients.  Each client's
  * key sits in one slot of a device pool of equally shaped keys; every item
