@@ -13,13 +13,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbslstm_tfhe.so")
-SOURCES = ["capi.hip", "blind_rotate.hip", "pbs_ops.hip", "lstm_ops.hip", "ks_mfma.hip"]
+SOURCES = ["capi.hip", "blind_rotate.hip", "pbs_ops.hip", "lstm_ops.hip", "ks_mfma.hip", "split_br.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-fPIC",
-    "-std=c++17",
+    "-std=c++20",
     # packed FP32 ops stay off in all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(HERE, "..", "include"),

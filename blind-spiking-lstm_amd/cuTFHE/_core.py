@@ -62,6 +62,11 @@ SIGNATURES = {
     "bslstm_key_switch_lwe_mfma_noalloc_max_bytes": ([], ctypes.c_int64),
     "bslstm_key_switch_lwe_mfma_noalloc": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64,
                                             _P, _P], ctypes.c_int),
+    "bslstm_mul_bootstrap_split_key_bytes": ([_P, _I64], ctypes.c_int64),
+    "bslstm_mul_bootstrap_split_prepare_key": ([_P, _P, _P, _I64, _I64, _P], ctypes.c_int),
+    "bslstm_mul_bootstrap_split_workspace_bytes": ([_I64, _P, _I64, _I64], ctypes.c_int64),
+    "bslstm_mul_bootstrap_split": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _I64, _I64, _I64, _I64,
+                                    _I64, _P, _I64], ctypes.c_int),
     "bslstm_mul_bootstrap_slots": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _I64, _P, _I64, _I64, _I64,
                                     _I64, _I64, _I64], ctypes.c_int),
     "bslstm_key_switch_lwe_mfma_slots": ([_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64,

@@ -56,6 +56,62 @@ def prepare_bsk(BSK: torch.Tensor, n: int, collapse: int) -> torch.Tensor:
 
 
 FORMS = ("plain", "batched", "rows")
+
+SPLIT_CHUNK = 512      # ciphertexts per chunk of the split form's step scratch
+
+
+def prepare_split_key(bsk_reshaped: torch.Tensor, l: int, degree: int) -> torch.Tensor:
+    """FFT-domain key (n_iter, 2, l, 2, m, N/2) complex128 -> the split
+    form's int8 Toeplitz limb rows (bslstm_mul_bootstrap_split_prepare_key).
+
+    The result is cached on the key tensor itself (an attribute that lives
+    and dies with it, tagged with the tensor's version so that a key written
+    in place is prepared again); a freed key's memory reused by a new tensor
+    can therefore never return a stale preparation."""
+    _core.require_device(bsk_reshaped)
+    bsk = bsk_reshaped.contiguous()
+    dims = tuple(int(d) for d in bsk.shape[-6:])
+    tag = (dims, bsk._version, l, degree)
+    hit = getattr(bsk_reshaped, "_bslstm_split_key", None)
+    if hit is not None and hit[0] == tag:
+        return hit[1]
+    arr = (_core._I64 * 6)(*dims)
+    nbytes = int(_core.lib().bslstm_mul_bootstrap_split_key_bytes(arr, degree))
+    if nbytes < 0:
+        raise _core.TFHEInvalidArgument(f"the split form does not take this key {dims} at N={degree}")
+    key = torch.empty(nbytes, dtype=torch.int8, device=bsk.device)
+    _core.check(_core.lib().bslstm_mul_bootstrap_split_prepare_key(
+        _core.stream_of(bsk), bsk.data_ptr(), arr, l, degree, key.data_ptr()))
+    bsk_reshaped._bslstm_split_key = (tag, key)
+    return key
+
+
+def mul_bootstrap_split(ct_init: torch.Tensor, split_key: torch.Tensor, bsk_dims, indices: torch.Tensor,
+                        log_beta: int, log_q: int, l: int, chunk: int = SPLIT_CHUNK) -> torch.Tensor:
+    """Exact blind rotation on the matrix cores (bslstm_mul_bootstrap_split):
+    same operands and results as mul_bootstrap with the key prepared by
+    prepare_split_key; equals the int128 product bit for bit (log_q <= 32)."""
+    _core.require_device(ct_init, split_key, indices)
+    if ct_init.dtype != torch.int64 or indices.dtype != torch.int32:
+        raise _core.TFHEInvalidArgument("ct_init must be int64 and indices int32")
+    degree = ct_init.shape[-1]
+    ct = ct_init.contiguous()
+    idx = indices.contiguous()
+    dims = [int(d) for d in bsk_dims]
+    n_iter, m = dims[0], dims[4]
+    batch = idx.numel() // (n_iter * m) if n_iter * m else 0
+    out = torch.empty(batch, 2, degree, dtype=torch.int64, device=ct.device)
+    arr = (_core._I64 * 6)(*dims)
+    ws_bytes = int(_core.lib().bslstm_mul_bootstrap_split_workspace_bytes(batch, arr, degree, chunk))
+    if ws_bytes < 0:
+        raise _core.TFHEInvalidArgument(f"the split form does not take key dims {dims} at N={degree}")
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.int8, device=ct.device)
+    _core.check(_core.lib().bslstm_mul_bootstrap_split(
+        _core.stream_of(ct), ct.data_ptr(), ct.numel(), split_key.data_ptr(), arr, idx.data_ptr(),
+        idx.numel(), out.data_ptr(), batch, log_beta, log_q, l, degree, ws.data_ptr(), ws_bytes))
+    return out
+
+
 _FORM_CODE = {"plain": 1, "batched": 2, "rows": 3}
 
 # Launch-shape autotuner: the fastest form per (degree, l, n_iter, m, log2
@@ -85,6 +141,8 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     shape); "batched": two ciphertexts per workgroup sharing every key load;
     "rows": one ciphertext per workgroup with its two accumulator rows on two
     thread groups (the latency form for small batches);
+    "split": exact integer products as int8 MFMA GEMMs (log_q <= 32; the key
+    is prepared once per key tensor and cached), equal to the exact product;
     "auto": the autotuner's choice for this shape (timed on first use; inside
     a HIP-graph capture an untuned shape takes "batched").  All forms give
     bit-identical results.
@@ -116,8 +174,14 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
                 times[f] = a.elapsed_time(b)
             _TUNED[key] = min(times, key=times.get)
             return out
+    if form == "split":
+        if slots is not None:
+            raise _core.TFHEInvalidArgument("the split form takes one key (no slots)")
+        degree = ct_init.shape[-1]
+        key = prepare_split_key(bsk_reshaped, l, degree)
+        return mul_bootstrap_split(ct_init, key, bsk_reshaped.shape[-6:], indices, log_beta, log_q, l)
     if form not in FORMS:
-        raise _core.TFHEInvalidArgument(f"form must be one of {FORMS} or 'auto'")
+        raise _core.TFHEInvalidArgument(f"form must be one of {FORMS}, 'split' or 'auto'")
     devs = [ct_init, bsk_reshaped, indices] + ([all_rot_fft] if all_rot_fft is not None else [])
     _core.require_device(*devs)
     if ct_init.dtype != torch.int64 or indices.dtype != torch.int32:

@@ -100,6 +100,41 @@ int bslstm_mul_bootstrap_rows(void *stream,
                               int64_t *output, int64_t batch,
                               int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree);
 
+/* ------------------------------------------------------------------------
+ * Blind rotation in exact integer arithmetic on the matrix cores (the
+ * "split" form): the same computation as bslstm_mul_bootstrap (the handler
+ * csrc/mul_bootstrap.cuh:192-243, kernel :20-136), with every negacyclic
+ * product exact -- the bootstrapping key cut into signed byte limbs, the
+ * batched products int8 MFMA GEMMs with int32 accumulation -- so outputs
+ * equal the exact (int128) product mod q bit for bit.  Needs log2_q <= 32,
+ * log2_beta <= 7 and degree 512, 1024 or 2048.
+ * ------------------------------------------------------------------------ */
+
+/* Bytes of the prepared key for bsk_dims (n_iter, 2, l, 2, m, N/2) at this
+ * degree (-1 on bad arguments). */
+int64_t bslstm_mul_bootstrap_split_key_bytes(const int64_t bsk_dims[6], int64_t degree);
+
+/* The FFT-domain key of bslstm_mul_bootstrap (complex128 (n_iter, 2, l, 2,
+ * m, N/2), multiply.py:81-86 layout; its integer coefficients are recovered
+ * exactly) -> split_key of bslstm_mul_bootstrap_split_key_bytes() bytes.
+ * Once per key. */
+int bslstm_mul_bootstrap_split_prepare_key(void *stream, const double *bsk,
+                                           const int64_t bsk_dims[6], int64_t l, int64_t degree,
+                                           int8_t *split_key);
+
+/* Workspace for a batch processed in chunks of up to `chunk` ciphertexts
+ * (chunk <= 0: 512); a larger workspace only allows larger chunks. */
+int64_t bslstm_mul_bootstrap_split_workspace_bytes(int64_t batch, const int64_t bsk_dims[6],
+                                                   int64_t degree, int64_t chunk);
+
+/* bslstm_mul_bootstrap's operands, attributes and errors with split_key in
+ * place of the FFT-domain key (all_rot_fft is not needed) and a workspace. */
+int bslstm_mul_bootstrap_split(void *stream, const int64_t *ct_init, int64_t ct_elems,
+                               const int8_t *split_key, const int64_t bsk_dims[6],
+                               const int32_t *indices, int64_t idx_elems, int64_t *output,
+                               int64_t batch, int64_t log2_beta, int64_t log2_q, int64_t l,
+                               int64_t degree, void *workspace, int64_t workspace_bytes);
+
 /*
  * Rotation exponents of the collapsed blind rotation.
  * Replaces the index preparation of cuTFHE/multiply.py:67-79 (an XLA fusion

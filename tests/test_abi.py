@@ -116,3 +116,31 @@ def test_lstm_path_entry_points_reject_on_host():
     assert lib.bslstm_key_switch_lwe_i64(None, 8, 16, 8, 8, 4, 2048, 702, 1, 20, 64, 8, 16) == bad
     assert _core.lib().bslstm_packing_workspace_bytes(2, 2048, 2048) == \
         2 * 2048 * 2048 * 8 + 2 * 256 * 2 * 1024 * 16
+
+
+def test_split_form_entry_points_reject_on_host():
+    """Validation of the split-form entry points runs before any GPU work."""
+    lib = _core.lib()
+    bad = _core.BSLSTM_INVALID_ARGUMENT
+    dims = (_core._I64 * 6)(250, 2, 3, 2, 8, 1024)
+    B = 16
+    ws = lib.bslstm_mul_bootstrap_split_workspace_bytes(B, dims, 2048, 0)
+    # 64 ciphertexts per workgroup at N = 2048: the smallest chunk
+    assert ws == 64 * (64 * 2 * 48 * 64 + 2 * 64 * 64 * 8)
+    assert lib.bslstm_mul_bootstrap_split_key_bytes(dims, 2048) == 250 * 64 * 48 * 2 * 5 * 128
+    assert lib.bslstm_mul_bootstrap_split_key_bytes(dims, 4096) == -1
+
+    def call(**kw):
+        a = dict(ct_elems=4096, idx_elems=B * 250 * 8, batch=B, lb=7, lq=32, l=3, degree=2048,
+                 ws_bytes=ws)
+        a.update(kw)
+        return lib.bslstm_mul_bootstrap_split(None, 8, a["ct_elems"], 8, dims, 8, a["idx_elems"], 8,
+                                              a["batch"], a["lb"], a["lq"], a["l"], a["degree"], 8,
+                                              a["ws_bytes"])
+    assert call(lq=64) == bad and "log2_q <= 32" in lib.bslstm_last_error().decode()
+    assert call(lb=8) == bad and "int8 digits" in lib.bslstm_last_error().decode()
+    assert call(degree=4096) == bad and "Unsupported degree" in lib.bslstm_last_error().decode()
+    assert call(l=2) == bad
+    assert call(idx_elems=5) == bad and "indices" in lib.bslstm_last_error().decode()
+    assert call(ws_bytes=ws - 1) == bad and "workspace" in lib.bslstm_last_error().decode()
+    assert call(batch=0, idx_elems=0) == _core.BSLSTM_OK
