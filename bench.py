@@ -28,14 +28,16 @@ CFG = dict(B=16384, n=750, N=2048, L=3, log_beta=7, collapse=3, n_lut=4, msg_bit
            sigma_lwe=2.0 ** 17, sigma_bsk=2.0 ** 5)
 
 FP64_PEAK_TFLOPS = 78.6
+INT8_PEAK_TOPS = 5000.0          # dense int8 MFMA
 
 # HBM bytes per launch of the blind-rotation kernel at this workload (16384
 # ciphertexts), from rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000), the
 # scales read off a 1 GiB torch.neg calibration kernel in the same runs
-# (profiles/br_kernel_2048_3_pbs16384.md; batched: mean of both launches,
-# plain: of the two after the first).
+# (batched: mean of both launches, plain: of the two after the first).
 # None for other batch sizes.
 HBM_TRAFFIC_BYTES = {"batched": 319.79e9 + 1.242e9, "plain": 202.2e9 + 0.604e9}
+HBM_TRAFFIC_SOURCE = ("profiles/br_kernel_2048_3_pbs16384.md (rocprofv3 --pmc FETCH_SIZE x2 + "
+                      "WRITE_SIZE x1, separate runs of bench.py at this workload)")
 
 
 def fp64_flops_per_bootstrap(n, N, L, c):
@@ -44,6 +46,16 @@ def fp64_flops_per_bootstrap(n, N, L, c):
     16*l*N*(2^c+1)]."""
     M = N // 2
     return (n // c) * ((2 * L + 2) * 5 * M * math.log2(M) + 16 * L * N * (2 ** c + 1))
+
+
+def split_int8_ops_per_bootstrap(n, N, L, c):
+    """Int8 MFMA work of one split-form blind rotation: per step and bin
+    (2R bins of Z[Y]/(Y^M+1), N = M*R) a (2M x P*M) Toeplitz-key by (P*M)
+    digit GEMM column, P = 2l*2^c rotated digit polynomials, for 9 limb
+    pairs; 2 operations per multiply-accumulate."""
+    M, R = {512: (32, 16), 1024: (32, 32), 2048: (64, 32)}[N]
+    P = 2 * L * 2 ** c
+    return (n // c) * (2 * R) * (2 * M) * (P * M) * 9 * 2
 
 
 MANY_LUT_FNS = (
@@ -95,7 +107,8 @@ def run_gpu(args):
     import numpy as np
     import torch
     sys.path.insert(0, PKG)
-    from cuTFHE.multiply import mul_bootstrap, rotation_indices
+    from cuTFHE.multiply import (mul_bootstrap, mul_bootstrap_split, prepare_split_key,
+                                 rotation_indices)
     from schemas.RLWE_jax import rotate_extract
 
     cfg = dict(CFG)
@@ -105,12 +118,20 @@ def run_gpu(args):
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
 
+    split_key = None
+    if args.split:
+        split_key = prepare_split_key(inputs["bsk"], cfg["L"], cfg["N"])
+
     def step(form, br_events=None):
         idx = rotation_indices(inputs["pk"], cfg["log_q"], cfg["N"], cfg["collapse"], cfg["n_lut"])
         if br_events is not None:
             br_events[0].record(stream)
-        br = mul_bootstrap(inputs["lut"], inputs["bsk"], None, idx, cfg["log_beta"], cfg["log_q"],
-                           cfg["L"], form=form)
+        if form == "split":
+            br = mul_bootstrap_split(inputs["lut"], split_key, inputs["bsk"].shape, idx,
+                                     cfg["log_beta"], cfg["log_q"], cfg["L"])
+        else:
+            br = mul_bootstrap(inputs["lut"], inputs["bsk"], None, idx, cfg["log_beta"],
+                               cfg["log_q"], cfg["L"], form=form)
         if br_events is not None:
             br_events[1].record(stream)
         return rotate_extract(br, inputs["b"], cfg["log_q"], cfg["n_lut"])
@@ -145,6 +166,25 @@ def run_gpu(args):
     else:
         out, ms_per_step, br_avg_ms = step("plain"), ms_plain, br_plain
     value = B / (ms_per_step / 1e3)
+    if args.split:
+        sout, ms_split, br_split = timed("split", 1, 1)
+        ops = split_int8_ops_per_bootstrap(cfg["n"], cfg["N"], cfg["L"], cfg["collapse"]) * B
+        ach = ops / (br_split / 1e3) / 1e12
+        forms["split_int8"] = {
+            "bootstraps_per_s": round(B / (ms_split / 1e3), 2), "ms_per_step": round(ms_split, 3),
+            "form_avg_ms": round(br_split, 3),
+            "equals_batched_fp64": bool(torch.equal(sout[0], out[0]) and torch.equal(sout[1], out[1])),
+            "roofline": {"bound": "int8-mfma", "achieved": round(ach, 1), "peak": INT8_PEAK_TOPS,
+                         "unit": "TOPS", "frac": round(ach / INT8_PEAK_TOPS, 4),
+                         "note": "algorithmic int8 MFMA operations over the whole split form "
+                                 "(GEMM and transform kernels)"}}
+        if "plain_fp64" in forms:
+            forms["split_over_plain"] = round(forms["split_int8"]["bootstraps_per_s"] /
+                                              forms["plain_fp64"]["bootstraps_per_s"], 3)
+        if "batched_fp64" in forms and "plain_fp64" in forms:
+            forms["batched_over_plain"] = round(forms["batched_fp64"]["bootstraps_per_s"] /
+                                                forms["plain_fp64"]["bootstraps_per_s"], 3)
+        split_key = None
 
     # decrypt a sample of the outputs on the host (sanity, not the parity test)
     masks, bodies = out
@@ -160,7 +200,7 @@ def run_gpu(args):
     flops = fp64_flops_per_bootstrap(cfg["n"], cfg["N"], cfg["L"], cfg["collapse"]) * B
     achieved = flops / (br_avg_ms / 1e3) / 1e12
     res = {
-        "metric": "bootstraps/s @N=2048 (batched many-LUT PBS)",
+        "metric": "bootstraps/s @N=2048 (split vs plain FP64-FFT)",
         "value": round(value, 2),
         "unit": "bootstraps/s",
         "n_gpus": 1,
@@ -172,12 +212,14 @@ def run_gpu(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded keys and ciphertexts)",
-        "config": {"workload": "batched many-LUT PBS", "batch": B, "n": cfg["n"], "N": cfg["N"],
+        "config": {"workload": "batched many-LUT PBS (value: the fastest form, batched FP64)",
+                   "batch": B, "n": cfg["n"], "N": cfg["N"],
                    "l": cfg["L"], "log2_Bg": cfg["log_beta"], "collapse": cfg["collapse"],
                    "n_lut": cfg["n_lut"], "torus": cfg["log_q"], "form": main_form + "_fp64"},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "fp64-valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": (HBM_TRAFFIC_BYTES[main_form] if B == CFG["B"] else None),
+                     "traffic_source": HBM_TRAFFIC_SOURCE,
                      "kernel": f"br_kernel<2048,3,{1 if main_form == 'plain' else 2},1,1,8>",
                      "kernel_avg_ms": round(br_avg_ms, 3)},
         "decrypt_ok_sample": ok_frac,
@@ -227,6 +269,103 @@ def run_form_sweep():
 # one encrypted spiking-LSTM layer at the reference's parameters
 # (parameters_crypto.py: q = 2^64, N = 2048, n = 702, l_bs = 2, Bg = 2^13,
 # c = 3; key switch 2^1 x 20; packing 2^8 x 4; trace packing 2^2 x 20)
+# ---------------------------------------------------------------------------
+# LSTM-path helpers: the synthetic layer of the one-layer workload and
+# decryption checks against the plaintext recurrence (nn_tfhe/plaintext.py)
+# ---------------------------------------------------------------------------
+LSTM_S = 4.0          # LUT half-range: pre-activations in [-4, 4)
+LSTM_W_SCALE = 16     # pre-activation = (hx @ W) / 16
+MARGIN_SIGMAS = 6.0   # determined units: this many LUT-index noise std from any discontinuity
+
+
+def synthetic_layer(rng, heavy, input_dim, hidden, n_lut, max_len):
+    """int4 gate weights U[-7, 7] scaled so that a unit weight moves a
+    pre-activation by 1/16 (the role of beta_w/s in setup_crypto.py:39-67),
+    thresholds U[-2, 2], eta = 1.  Returns (CipherSpikeLSTM, plaintext model)."""
+    import numpy as np
+    import parameters_crypto as PC
+    import setup_crypto as SC
+    from nn_tfhe.Spike_LSTM import CipherSpikeLSTM
+    from nn_tfhe.plaintext import PlainSpikingLSTM
+    G, K = hidden // n_lut, input_dim + hidden
+    factor = PC.q / LSTM_W_SCALE / LSTM_W_SCALE / PC.beta_x
+    W = {c: rng.integers(-7, 8, (K, G)) for c in "fio"}
+    lin = [SC.make_linear(W[c], np.zeros(G), factor, PC.dict_params, device=heavy.device)
+           for c in "fio"]
+    gam = {c: rng.uniform(-2, 2, (n_lut, G)) for c in "fio"}
+    eta = np.ones((n_lut, G), np.int64)
+    sv = np.full(G, LSTM_S)
+    lut_f, lut_i, lut_o = SC.get_lut(n_lut, hidden, PC.degree_lut, sv, sv, sv, gam["f"], gam["i"],
+                                     gam["o"], eta)
+    lstm = CipherSpikeLSTM(PC.dict_params, PC.dict_params_lut, input_dim, hidden, max_len, lin[0],
+                           lin[1], lin[2], heavy, n_lut, lut_f, lut_i, lut_o, PC.beta_x)
+    plain = PlainSpikingLSTM(W, {c: np.zeros(G) for c in "fio"}, gam, eta, LSTM_S, n_lut,
+                             LSTM_W_SCALE, int(PC.q / PC.beta_x))
+    return lstm, plain
+
+
+def lstm_margin(n_lut):
+    import parameters_crypto as PC
+    from nn_tfhe.plaintext import lut_index_noise
+    sigma = lut_index_noise(PC.dict_params_lut, PC.dict_params_ks_LWE, PC.degree, PC.q, n_lut)
+    return MARGIN_SIGMAS * sigma * 2 * LSTM_S / (PC.degree_lut // n_lut), sigma
+
+
+def decrypt_values(ct, sk, lo, hi):
+    """Decrypted values (units of beta_x, mod q/beta_x) of coefficients lo:hi."""
+    import numpy as np
+    import parameters_crypto as PC
+    from nn_tfhe.plaintext import wrap
+    from schemas.RLWE_jax import _phase
+    ph = _phase(ct, sk, PC.dict_params)
+    return wrap(np.rint(np.asarray(ph)[..., lo:hi] / PC.beta_x).astype(np.int64),
+                int(PC.q / PC.beta_x))
+
+
+def count_determined(plain, margin, x, Hdec, Cdec):
+    """Per step, the plaintext rule on the previous decrypted states:
+    (determined units, of those decrypting to the rule, all units)."""
+    import numpy as np
+    H, C = np.zeros_like(Hdec[0]), np.zeros_like(Cdec[0])
+    det_n = ok_n = tot = 0
+    for t in range(x.shape[0]):
+        Hp, Cp, det = plain.step(x[t], H, C, margin)
+        det_n += int(det.sum())
+        ok_n += int((det & (Hdec[t] == Hp) & (Cdec[t] == Cp)).sum())
+        tot += det.size
+        H, C = Hdec[t], Cdec[t]
+    return det_n, ok_n, tot
+
+
+def lstm_decrypt_check(lstm, plain, sk, S, steps, rng):
+    """`steps` timesteps of S sequences with fresh spike inputs; every step's
+    states decrypted and compared with the plaintext rule."""
+    import numpy as np
+    import torch
+    import parameters_crypto as PC
+    from schemas.RLWE_jax import encrypt
+    inp, hid, N = lstm.input_dim, lstm.hidden_dim, lstm.degree
+    x = rng.integers(0, 2, (steps, S, inp))
+    msg = np.zeros((steps, S, N), np.int64)
+    msg[..., :inp] = x.astype(np.int64) << 56
+    A, b = encrypt(msg, sk, PC.dict_params, rng)
+    dev = lstm.heavyBS.device
+    xs = torch.stack([torch.as_tensor(A), torch.as_tensor(b)], dim=2).to(dev)
+    H = torch.zeros(S, 2, N, dtype=torch.int64, device=dev)
+    C = torch.zeros_like(H)
+    lut = lstm.forget_lut(C)
+    Hd, Cd = [], []
+    for t in range(steps):
+        H, C, lut = lstm.step_pipelined(H, C, xs[t], lut)
+        Hd.append(decrypt_values((H[:, 0].cpu().numpy(), H[:, 1].cpu().numpy()), sk, inp, inp + hid))
+        Cd.append(decrypt_values((C[:, 0].cpu().numpy(), C[:, 1].cpu().numpy()), sk, 0, hid))
+    margin, sigma = lstm_margin(lstm.n_lut)
+    det, ok, tot = count_determined(plain, margin, x, np.stack(Hd), np.stack(Cd))
+    return {"sequences": S, "steps": steps, "units": tot, "determined_frac": round(det / tot, 4),
+            "determined_correct_frac": round(ok / max(det, 1), 6),
+            "lut_index_noise_entries": round(sigma, 2), "margin_sigmas": MARGIN_SIGMAS}
+
+
 LSTM_CFG = dict(input_dim=128, hidden=128, n_lut=4, seed=0)
 
 
@@ -257,20 +396,14 @@ def run_lstm(args):
     G = hid // nl
     heavy, _ = SC.get_tfhe_activation(bsk, ksk_bs, ksk_packing, galois_key, np.ones(G), nl,
                                       device=dev)
-    # int4 weights U[-7, 7] scaled so that a unit weight moves a pre-activation
-    # by 1/16 of the LUT range (the role of beta_w/s in setup_crypto.py:39-67)
-    factor = PC.q / 16 / 16 / PC.beta_x
-    lin = [SC.make_linear(rng.integers(-7, 8, (K, G)), np.zeros(G), factor, PC.dict_params,
-                          device=dev) for _ in range(3)]
-    s = np.full(G, 4.0)
-    gam = [rng.uniform(-2, 2, (nl, G)) for _ in range(3)]
-    lut_f, lut_i, lut_o = SC.get_lut(nl, hid, PC.degree_lut, s, s, s, gam[0], gam[1], gam[2],
-                                     np.ones((nl, G)))
-    lstm = CipherSpikeLSTM(PC.dict_params, PC.dict_params_lut, cfg["input_dim"], hid, 128, lin[0],
-                           lin[1], lin[2], heavy, nl, lut_f, lut_i, lut_o, PC.beta_x)
+    lstm, plain = synthetic_layer(rng, heavy, cfg["input_dim"], hid, nl, 128)
     out = {"workload": f"one encrypted spiking-LSTM layer: input {cfg['input_dim']}, hidden {hid}, "
-                       f"n_lut {nl}, q=2^64, N=2048, n=702, l=2, Bg=2^13, c=3",
+                       f"n_lut {nl}, 3 gates (f, i, o: the reference's spiking cell), at the "
+                       f"reference's LSTM parameters q=2^64, N=2048, n=702, l=2, Bg=2^13, c=3 "
+                       f"(not the torus32 PBS parameters)",
            "keygen_s": round(keygen_s, 2), "bootstraps_per_step_per_sequence": 3 * G}
+    for S in (1, 4):
+        out[f"decrypt_check_batch{S}"] = lstm_decrypt_check(lstm, plain, sk, S, 3, rng)
     N = PC.degree
     for S, steps in ((1, args.lstm_steps), (args.lstm_batch, max(4, args.lstm_steps // 4))):
         if steps <= 0 or S <= 0:
@@ -350,37 +483,32 @@ def run_classifier(args):
     G = hid // nl
     heavy, relu = SC.get_tfhe_activation(bsk, ksk_bs, ksk_packing, galois_key,
                                          np.full(cfg["head_hidden"], 4.0), nl, device=dev)
-    factor = PC.q / 16 / 16 / PC.beta_x
-    lin = [SC.make_linear(rng.integers(-7, 8, (K, G)), np.zeros(G), factor, PC.dict_params,
-                          device=dev) for _ in range(3)]
-    s4 = np.full(G, 4.0)
-    gam = [rng.uniform(-2, 2, (nl, G)) for _ in range(3)]
-    lut_f, lut_i, lut_o = SC.get_lut(nl, hid, PC.degree_lut, s4, s4, s4, gam[0], gam[1], gam[2],
-                                     np.ones((nl, G)))
-    lstm = CipherSpikeLSTM(PC.dict_params, PC.dict_params_lut, cfg["input_dim"], hid, cfg["len_hi"],
-                           lin[0], lin[1], lin[2], heavy, nl, lut_f, lut_i, lut_o, PC.beta_x)
-    # int8 head weights
-    head1 = SC.make_linear(rng.integers(-127, 128, (hid, cfg["head_hidden"])),
-                           np.zeros(cfg["head_hidden"]), 1.0, PC.dict_params, device=dev)
-    head2 = SC.make_linear(rng.integers(-127, 128, (cfg["head_hidden"], cfg["classes"])),
-                           np.zeros(cfg["classes"]), 1.0, PC.dict_params, device=dev)
+    lstm, plain = synthetic_layer(rng, heavy, cfg["input_dim"], hid, nl, cfg["len_hi"])
+    # int8 head weights in fixed point (x 1/64, x 1/128): the ReLU inputs stay
+    # inside the LUT range [-4, 4) and the logits inside the torus
+    W1 = rng.integers(-127, 128, (hid, cfg["head_hidden"]))
+    W2 = rng.integers(-127, 128, (cfg["head_hidden"], cfg["classes"]))
+    head1 = SC.make_linear(W1, np.zeros(cfg["head_hidden"]), 1 / 64, PC.dict_params, device=dev)
+    head2 = SC.make_linear(W2, np.zeros(cfg["classes"]), 1 / 128, PC.dict_params, device=dev)
     head = CipherMLP(PC.dict_params, PC.dict_params_lut, hid, head1, head2, relu)
+    Wt1, Wt2 = np.rint(W1 / 64).astype(np.int64), np.rint(W2 / 128).astype(np.int64)
     # client side: token ids, binarized embeddings, encryption
     emb = rng.standard_normal((cfg["vocab"], cfg["input_dim"])) > 0
     lens = rng.integers(cfg["len_lo"], cfg["len_hi"] + 1, cfg["sequences"])
-    seqs = []
+    seqs, xs_plain = [], []
     for L_ in lens:
         tok = np.minimum(rng.zipf(cfg["zipf"], L_), cfg["vocab"]) - 1
         msg = np.zeros((L_, PC.degree), np.int64)
         msg[:, :cfg["input_dim"]] = emb[tok].astype(np.int64) << 56
+        xs_plain.append(emb[tok].astype(np.int64))
         A, b = encrypt(msg, sk, PC.dict_params, rng)
         seqs.append((torch.as_tensor(A).to(dev), torch.as_tensor(b).to(dev)))
     shift = torch.full((1,), -cfg["input_dim"], dtype=torch.int64, device=dev)
     out = {"workload": f"encrypted spiking-LSTM classifier: {cfg['sequences']} sequences, lengths "
                        f"U[{cfg['len_lo']},{cfg['len_hi']}], Zipf({cfg['zipf']}) tokens over "
                        f"{cfg['vocab']}, embedding {cfg['input_dim']} -> spikes, hidden {hid}, "
-                       f"n_lut {nl}, head {hid}->{cfg['head_hidden']}->ReLU->{cfg['classes']}, "
-                       f"q=2^64, N=2048",
+                       f"n_lut {nl}, 3 gates, head {hid}->{cfg['head_hidden']}->ReLU->"
+                       f"{cfg['classes']} (int8 weights x 1/64, x 1/128), q=2^64, N=2048",
            "total_timesteps": int(lens.sum()),
            "bootstraps": int(lens.sum()) * 3 * G + cfg["sequences"] * cfg["head_hidden"]}
     for use_graph in (False, True):
@@ -400,13 +528,63 @@ def run_classifier(args):
             "step_p50_ms": round(float(np.percentile(ms, 50)), 3),
             "step_p99_ms": round(float(np.percentile(ms, 99)), 3), "steps": len(ms)}
     out["logits_shape"] = list(logits[0].shape)
+    out["decrypt_check"] = classifier_decrypt_check(lstm, plain, head, Wt1, Wt2, sk, seqs, lens,
+                                                    xs_plain, H, logits, cfg)
     out["tuned_forms"] = {str(k): v for k, v in tuned_forms().items()}
     return out
 
 
+def classifier_decrypt_check(lstm, plain, head, Wt1, Wt2, sk, seqs, lens, xs_plain, H, logits, cfg):
+    """(a) The two shortest sequences re-run step by step (eager), every
+    step's states decrypted against the plaintext rule, and their final
+    states equal to the engine's; (b) every sequence's logits decrypted and
+    compared with the plaintext head applied to its decrypted final hidden
+    state (ReLU ramp within the noise bound, logits within the bound
+    propagated through the second Linear, predicted class)."""
+    import numpy as np
+    import torch
+    import parameters_crypto as PC
+    from nn_tfhe.plaintext import fold, lut_index_noise, relu_ramp
+    from schemas.LWE_jax import lwe_phase
+    from schemas.text_jax import rotate_ciphertext
+    inp, hid = cfg["input_dim"], cfg["hidden"]
+    margin, _ = lstm_margin(cfg["n_lut"])
+    det = ok = tot = 0
+    same = True
+    for i in np.argsort(lens)[:2]:
+        (H1, C1), (Hs, Cs) = lstm(seqs[i])
+        same &= bool(torch.equal(H[i, 0], H1[0]) and torch.equal(H[i, 1], H1[1]))
+        Hd = decrypt_values((Hs[0].cpu().numpy(), Hs[1].cpu().numpy()), sk, inp, inp + hid)
+        Cd = decrypt_values((Cs[0].cpu().numpy(), Cs[1].cpu().numpy()), sk, 0, hid)
+        d, o, t = count_determined(plain, margin, xs_plain[i], Hd, Cd)
+        det, ok, tot = det + d, ok + o, tot + t
+    # head: plaintext on the decrypted final H
+    shift = torch.full((1,), -inp, dtype=torch.int64, device=H.device)
+    Hr = rotate_ciphertext((H[:, 0], H[:, 1]), shift)
+    Hv = decrypt_values((Hr[0].cpu().numpy(), Hr[1].cpu().numpy()), sk, 0, hid)
+    pre = (Hv @ Wt1) / LSTM_W_SCALE
+    step = LSTM_S / (PC.degree_lut // 2)
+    sig = lut_index_noise(PC.dict_params_lut, PC.dict_params_ks_LWE, PC.degree, PC.q, 1) * step
+    ok_relu = np.abs(np.abs(fold(pre, LSTM_S)) - LSTM_S) >= MARGIN_SIGMAS * sig
+    y_plain = relu_ramp(pre, LSTM_S, PC.degree_lut // 2)
+    lg_plain = y_plain @ Wt2
+    lg = lwe_phase((logits[0].cpu().numpy(), logits[1].cpu().numpy()), sk, PC.dict_params) / PC.beta_x
+    lg_sig = sig * np.sqrt((Wt2 ** 2).sum(0))
+    rows = ok_relu.all(axis=1)
+    close = np.abs(lg - lg_plain) <= MARGIN_SIGMAS * lg_sig
+    return {"sequences_stepped": 2, "units": tot, "determined_frac": round(det / max(tot, 1), 4),
+            "determined_correct_frac": round(ok / max(det, 1), 6),
+            "engine_equals_eager": same,
+            "logits_rows_determined": int(rows.sum()),
+            "logits_within_noise_frac": round(float(close[rows].mean()) if rows.any() else 0.0, 4),
+            "class_agreement_frac": round(float((lg.argmax(1) == lg_plain.argmax(1))[rows].mean())
+                                          if rows.any() else 0.0, 4),
+            "relu_noise_sigma": round(float(sig), 4)}
+
+
 # serving: Poisson arrivals of single-sequence requests (lengths U[4, 128])
 # from 64 clients with their own keys; one LSTM layer (input 128, hidden 128)
-SERVE_CFG = dict(clients=64, slots=48, qps=4.0, duration_s=30.0, len_lo=4, len_hi=128,
+SERVE_CFG = dict(clients=64, reserve_gib=24, qps=4.0, duration_s=30.0, len_lo=4, len_hi=128,
                  input_dim=128, hidden=128, n_lut=4, max_active=96, seed=2)
 
 
@@ -445,7 +623,12 @@ def run_serving(args):
                  n_lut=nl, input_dim=cfg["input_dim"], hidden=hid, max_len=cfg["len_hi"],
                  linear_f=lin[0], linear_i=lin[1], linear_o=lin[2], lut_f=lut_f, lut_i=lut_i,
                  lut_o=lut_o, beta_x=PC.beta_x)
-    pool = KeyPool(cfg["slots"], keys[0], device=dev)
+    # admission by HBM footprint: as many resident key sets as free device
+    # memory holds beside a reserve for the step workspaces, at most one per client
+    slot_bytes = sum(v.numel() * v.element_size() for v in keys[0].values())
+    free_b, _ = torch.cuda.mem_get_info(dev)
+    slots = int(max(1, min(cfg["clients"], (free_b - cfg["reserve_gib"] * 2 ** 30) // slot_bytes)))
+    pool = KeyPool(slots, keys[0], device=dev)
     srv = EncryptedLSTMServer(margs, pool, lambda c: keys[c], max_active=cfg["max_active"])
     # warm-up: one short request per batch-size bucket tunes the launch shapes
     for S in (1, 2, 4, 8, 16, 32, 64, 96):
@@ -469,11 +652,13 @@ def run_serving(args):
     w0 = time.perf_counter()
     done = srv.run(reqs)
     wall = time.perf_counter() - w0
+    check = serving_decrypt_check(done, secrets, margs, keys, pool, dev)
     lat = np.array([r.done - r.arrival for r in done])
     per_step = np.array([(r.done - r.arrival) / r.x.shape[0] for r in done])
     return {"workload": f"continuous-batching server: Poisson {cfg['qps']} req/s for "
                         f"{cfg['duration_s']} s, {cfg['clients']} clients with own keys, "
-                        f"{cfg['slots']} key slots, lengths U[{cfg['len_lo']},{cfg['len_hi']}], "
+                        f"{slots} key slots (free HBM / {slot_bytes / 2 ** 20:.0f} MiB per "
+                        f"client), lengths U[{cfg['len_lo']},{cfg['len_hi']}], "
                         f"one LSTM layer input {cfg['input_dim']} hidden {hid}, q=2^64, N=2048",
             "requests": len(done), "timesteps": int(sum(r.x.shape[0] for r in done)),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
@@ -482,26 +667,56 @@ def run_serving(args):
             "p99_latency_per_timestep_ms": round(float(np.percentile(per_step, 99)) * 1e3, 2),
             "bootstraps_per_s": round(srv.bootstraps / wall, 1), "wall_s": round(wall, 2),
             "ticks": srv.ticks, "gpu_busy_frac": round(srv.gpu_busy_s() / wall, 3),
-            "key_loads": pool.loads,
+            "key_loads": pool.loads, "decrypt_check": check,
             "key_bytes_per_client": pool.slot_bytes, "key_pool_bytes": pool.footprint_bytes(),
             "client_keygen_s": round(keygen_s, 2)}
 
 
-def cpu_baseline_child(sample: int):
+def serving_decrypt_check(done, secrets, margs, keys, pool, dev):
+    """Every finished request's final cell state decrypted under its own
+    client's key: integer values (units of beta_x) of magnitude at most the
+    sequence length (the cell state counts input spikes); and the three
+    shortest requests served again alone, final states compared bit for bit
+    with the mixed run."""
+    import numpy as np
+    import torch
+    import parameters_crypto as PC
+    from nn_tfhe.serving import EncryptedLSTMServer, KeyPool, Request
+    from schemas.RLWE_jax import _phase
+    hid = margs["hidden"]
+    ok = 0
+    for r in done:
+        ph = _phase((r.result[1][0].cpu().numpy(), r.result[1][1].cpu().numpy()),
+                    secrets[r.client][0], PC.dict_params)[:hid] / PC.beta_x
+        ok += int(np.abs(ph - np.rint(ph)).max() < 0.05 and
+                  np.abs(np.rint(ph)).max() <= r.x.shape[0])
+    same = 0
+    short = sorted(done, key=lambda r: r.x.shape[0])[:3]
+    for r in short:
+        solo_pool = KeyPool(1, keys[r.client], device=dev)
+        solo = EncryptedLSTMServer(margs, solo_pool, lambda c: keys[c], max_active=1)
+        fin = solo.run([Request(r.rid, r.client, r.x, 0.0)], clock=lambda: float(solo.ticks))
+        same += int(torch.equal(fin[0].result[0], r.result[0]) and
+                    torch.equal(fin[0].result[1], r.result[1]))
+    return {"requests": len(done), "cell_states_integral_frac": round(ok / max(len(done), 1), 4),
+            "alone_equals_mix": f"{same}/{len(short)}"}
+
+
+def cpu_baseline_child(sample: int, seed: int = 1):
     """Oracle (NumPy FP64 restatement of the reference algorithm) on a
     bounded sample of the same workload, single-threaded."""
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import tfhe_oracle as O
     cfg = CFG
-    rng = np.random.default_rng(1)
+    rng = np.random.default_rng(seed)
     N, n, L, c = cfg["N"], cfg["n"], cfg["L"], cfg["collapse"]
     n_iter = n // c
     m = 2 ** c
     # random FFT-domain key of the right shape: the timing does not depend on
     # the key values
-    bsk_k = (rng.standard_normal((n_iter, 2, L, 2, m, N // 2)) +
-             1j * rng.standard_normal((n_iter, 2, L, 2, m, N // 2))) * 2.0 ** 30
+    bsk_k = rng.standard_normal((n_iter, 2, L, 2, m, N // 2, 2)).view(np.complex128)[..., 0]
+    bsk_k *= 2.0 ** 30
     a = rng.integers(-2 ** 31, 2 ** 31, size=(sample, n))
     b = rng.integers(-2 ** 31, 2 ** 31, size=sample)
     lut = rng.integers(-2 ** 31, 2 ** 31, size=(sample, 2, N))
@@ -607,11 +822,15 @@ def run_pbs_keyswitch(cpu: bool):
     return out
 
 
-def precision_check(sample: int):
-    """Phase error of the GPU blind rotation against the exact int128 oracle
-    and against the NumPy FP64 restatement of the reference algorithm, on
-    `sample` ciphertexts of the batched-PBS workload with real seeded keys.
-    The oracle is the checker here; the GPU kernels are what is measured."""
+def precision_check(sample: int, restate_sample: int, oracle_sample: int):
+    """Phase error of the FP64 blind rotation (batched form) against exact
+    products, on `sample` ciphertexts of the batched-PBS workload with real
+    seeded keys.  The exact products at this size come from the split form
+    (exact integer arithmetic); the int128 oracle re-checks the first
+    `oracle_sample` of them, and the NumPy FP64 restatement of the reference
+    algorithm (the reference's own arithmetic, for its error) runs on the
+    first `restate_sample` (all host cores).  The oracle and the restatement
+    are the checkers; the GPU kernels are what is measured."""
     import numpy as np
     import torch
     sys.path.insert(0, ROOT)
@@ -647,7 +866,7 @@ def precision_check(sample: int):
     lut_t = torch.as_tensor(lut_b).to(dev)
     idx_t = rotation_indices(pk, lq, N, c, nl)
     gpu = {}
-    for form in ("batched", "plain"):
+    for form in ("batched", "plain", "split"):
         br = mul_bootstrap(lut_t, bsk_t, None, idx_t, lb, lq, L, form=form)
         masks, bodies = rotate_extract(br, bd, lq, nl)
         torch.cuda.synchronize()
@@ -655,56 +874,140 @@ def precision_check(sample: int):
                      bodies.cpu().numpy().astype(np.int64).reshape(sample, nl))
     del bsk_t
     idx = O.rotation_indices(a, lq, N, c, nl)
-    br_x = X.blind_rotate(lut, coef_k, idx, lb, lq, L)
-    br_f = np.asarray(O.blind_rotate_f64(lut_b, fft_k, idx, lb, lq, L), np.int64)
+    br_x = X.blind_rotate(lut, coef_k, idx[:oracle_sample], lb, lq, L)
+    br_f = restate_blind_rotate(lut, fft_k, idx[:restate_sample], lb, lq, L)
 
-    def extract(br):
-        m_, b_ = O.cuboot_merge_tail(br, b, lq, nl)
-        return np.asarray(m_, np.int64), np.asarray(b_, np.int64).reshape(sample, nl)
+    def extract(br, k):
+        m_, b_ = O.cuboot_merge_tail(br, b[:k], lq, nl)
+        return np.asarray(m_, np.int64), np.asarray(b_, np.int64).reshape(k, nl)
 
     expect = np.array([[f(int(x)) for f in MANY_LUT_FNS[:nl]] for x in msg], np.int64)
     mod = 1 << (mb + 1)
 
     def decrypt(masks, bodies):
+        k = masks.shape[0]
         ph = O.centered_mod(bodies - masks @ S.astype(np.int64), lq)
-        return np.mod(np.rint(ph / delta).astype(np.int64), mod), O.centered_mod(ph - expect * delta, lq)
+        return (np.mod(np.rint(ph / delta).astype(np.int64), mod),
+                O.centered_mod(ph - expect[:k] * delta, lq))
 
     dec_g, err_g = decrypt(*gpu["batched"][1:])
-    dec_x, err_x = decrypt(*extract(br_x))
-    dec_f, err_f = decrypt(*extract(br_f))
-    dev_g = O.centered_mod(gpu["batched"][0] - br_x, lq)
-    dev_f = O.centered_mod(br_f - br_x, lq)
+    dec_s, err_s = decrypt(*gpu["split"][1:])
+    dec_f, err_f = decrypt(*extract(br_f, restate_sample))
+    dev_g = O.centered_mod(gpu["batched"][0] - gpu["split"][0], lq)
+    dev_f = O.centered_mod(br_f - gpu["split"][0][:restate_sample], lq)
     pred = O.predicted_bs_std(N, n, lb, L, c, cfg["sigma_bsk"], lq)
+    R = restate_sample
     return {
         "sample": f"{sample} ciphertexts of the batched-PBS workload, seeded keys "
                   f"(sigma_bsk 2^{math.log2(cfg['sigma_bsk']):.0f}, sigma_lwe "
-                  f"2^{math.log2(cfg['sigma_lwe']):.0f})",
+                  f"2^{math.log2(cfg['sigma_lwe']):.0f}); exact products from the split form, "
+                  f"its first {oracle_sample} re-checked by the int128 oracle; the NumPy FP64 "
+                  f"restatement of the reference algorithm on the first {R}",
+        "split_equals_int128_oracle": bool(np.array_equal(gpu["split"][0][:oracle_sample], br_x)),
         "coef_dev_vs_exact": {"gpu_max_abs": int(np.abs(dev_g).max()), "gpu_std": float(dev_g.std()),
                               "restatement_max_abs": int(np.abs(dev_f).max()),
                               "restatement_std": float(dev_f.std())},
         "batched_equals_plain": bool(np.array_equal(gpu["batched"][0], gpu["plain"][0])),
-        "phase_err_std": {"gpu": float(err_g.std()), "exact": float(err_x.std()),
+        "phase_err_std": {"gpu": float(err_g.std()), "exact": float(err_s.std()),
+                          "gpu_on_restatement_sample": float(err_g[:R].std()),
                           "restatement": float(err_f.std()), "predicted_bs": pred},
-        "phase_err_std_ratio_gpu_vs_restatement": float(err_g.std() / err_f.std()),
-        "messages_identical_gpu_vs_exact": float((dec_g == dec_x).mean()),
-        "messages_identical_gpu_vs_restatement": float((dec_g == dec_f).mean()),
+        "phase_err_std_ratio_gpu_vs_restatement": float(err_g[:R].std() / err_f.std()),
+        "phase_err_std_ratio_gpu_vs_exact": float(err_g.std() / err_s.std()),
+        "messages_identical_gpu_vs_exact": float((dec_g == dec_s).mean()),
+        "messages_identical_gpu_vs_restatement": float((dec_g[:R] == dec_f).mean()),
         "messages_equal_lut": float((dec_g == np.mod(expect, mod)).mean()),
         "seconds": round(time.perf_counter() - t0, 1),
     }
 
 
-def cpu_baseline(sample: int):
+def _restate_child(path: str, lo: int, hi: int, lb: int, lq: int, L: int):
+    """Worker of restate_blind_rotate: rows lo:hi of the inputs saved at path."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import tfhe_oracle as O
+    d = np.load(path)
+    lut = np.ascontiguousarray(np.broadcast_to(d["lut"], (hi - lo,) + d["lut"].shape[-2:]))
+    br = np.asarray(O.blind_rotate_f64(lut, d["bsk"], d["idx"][lo:hi], lb, lq, L), np.int64)
+    np.save(path.replace(".npz", f".{lo}.npy"), br)
+
+
+def restate_blind_rotate(lut, fft_k, idx, lb, lq, L):
+    """The oracle's NumPy FP64 restatement of the reference's blind rotation
+    over all host cores (one single-threaded process per core)."""
+    import numpy as np
+    import tempfile
+    P = host_cores()
+    B = idx.shape[0]
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "restate.npz")
+        np.savez(path, lut=lut, bsk=fft_k, idx=idx)
+        bounds = [(B * i // P, B * (i + 1) // P) for i in range(P)]
+        procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--restate-child",
+                                   f"{path},{lo},{hi},{lb},{lq},{L}"], env=single_thread_env())
+                 for lo, hi in bounds if hi > lo]
+        for p in procs:
+            if p.wait() != 0:
+                raise RuntimeError("restatement worker failed")
+        return np.concatenate([np.load(path.replace(".npz", f".{lo}.npy"))
+                               for lo, hi in bounds if hi > lo])
+
+
+def host_cores() -> int:
+    """Host cores this process may use (at most 16, the GPU box's share)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def single_thread_env() -> dict:
     env = dict(os.environ)
     for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
         env[k] = "1"
-    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
-                        str(sample)], env=env, capture_output=True, text=True, timeout=600)
-    if r.returncode != 0:
-        return {"error": r.stderr[-400:]}
-    d = json.loads(r.stdout.strip().splitlines()[-1])
-    return {"value": round(d["value"], 4), "unit": "bootstraps/s", "cores": 1, "kind": "port",
+    return env
+
+
+def cpu_baseline(sample: int):
+    """The oracle's NumPy FP64 restatement of the reference algorithm on a
+    bounded sample of the headline workload: one single-threaded process
+    (1 core), then one process per host core at once (all cores)."""
+    def run(n_procs, per):
+        procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+                                   str(per), "--cpu-baseline-seed", str(1 + i)],
+                                  env=single_thread_env(), stdout=subprocess.PIPE, text=True)
+                 for i in range(n_procs)]
+        outs = []
+        for p in procs:
+            o, _ = p.communicate(timeout=900)
+            if p.returncode != 0:
+                raise RuntimeError("CPU baseline worker failed")
+            outs.append(json.loads(o.strip().splitlines()[-1]))
+        return outs
+
+    try:
+        one = run(1, sample)[0]
+        P = host_cores()
+        per = max(8, sample // 4)
+        many = run(P, per)
+        wall = max(d["seconds"] for d in many)
+        all_rate = P * per / wall
+    except (RuntimeError, subprocess.TimeoutExpired) as e:
+        return {"error": str(e)}
+    return {"value": round(one["value"], 4), "unit": "bootstraps/s", "cores": 1, "kind": "port",
+            "all_cores": {"value": round(all_rate, 3), "cores": P,
+                          "sample": f"{per} ciphertexts per process, {P} processes at once, "
+                                    f"slowest {wall:.1f} s"},
+            "cpu_model": cpu_model(),
             "sample": f"{sample} ciphertexts of the same workload through the NumPy FP64 "
-                      f"restatement of the reference algorithm (oracle), {d['seconds']:.1f} s, "
+                      f"restatement of the reference algorithm (oracle), {one['seconds']:.1f} s, "
                       f"1 thread"}
 
 
@@ -722,9 +1025,14 @@ def main():
                     help="also run one elementwise negation over this many MiB (reads and "
                          "writes that many bytes): the calibration kernel for HBM counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision-sample", type=int, default=16,
-                    help="ciphertexts compared against the exact int128 oracle and the FP64 "
-                         "restatement (phase-error statistics; 0: skip)")
+    ap.add_argument("--precision-sample", type=int, default=4096,
+                    help="ciphertexts whose FP64 blind rotation is compared with exact products "
+                         "(phase-error statistics; 0: skip)")
+    ap.add_argument("--precision-restate", type=int, default=256,
+                    help="of those, ciphertexts also run through the NumPy FP64 restatement of "
+                         "the reference algorithm (all host cores)")
+    ap.add_argument("--precision-oracle", type=int, default=16,
+                    help="of those, ciphertexts re-checked by the int128 oracle")
     ap.add_argument("--lstm-steps", type=int, default=32,
                     help="timesteps of the encrypted spiking-LSTM layer to time (0: skip)")
     ap.add_argument("--lstm-batch", type=int, default=32,
@@ -737,10 +1045,18 @@ def main():
                     help="run the 256-ciphertext N=1024 PBS + key switch case (0: skip)")
     ap.add_argument("--form-sweep", type=int, default=1,
                     help="time the three blind-rotation forms at several batch sizes (0: skip)")
+    ap.add_argument("--split", type=int, default=1,
+                    help="also time the split form (exact int8 MFMA) at the headline workload")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
+    ap.add_argument("--cpu-baseline-seed", type=int, default=1)
+    ap.add_argument("--restate-child", type=str, default="")
     args = ap.parse_args()
     if args.cpu_baseline_child:
-        cpu_baseline_child(args.cpu_baseline_child)
+        cpu_baseline_child(args.cpu_baseline_child, args.cpu_baseline_seed)
+        return
+    if args.restate_child:
+        p, lo, hi, lb, lq, L = args.restate_child.split(",")
+        _restate_child(p, int(lo), int(hi), int(lb), int(lq), int(L))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the engine has no "
@@ -764,7 +1080,9 @@ def main():
         res["copy_calibration_bytes"] = args.copy_mib * 2 ** 20
     res["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample)
     if not args.no_cpu_baseline and args.precision_sample > 0:
-        res["precision"] = precision_check(args.precision_sample)
+        res["precision"] = precision_check(args.precision_sample,
+                                           min(args.precision_restate, args.precision_sample),
+                                           min(args.precision_oracle, args.precision_sample))
     if args.pbs_keyswitch:
         res["pbs_keyswitch_n1024"] = run_pbs_keyswitch(cpu=not args.no_cpu_baseline)
     print(json.dumps(res), flush=True)

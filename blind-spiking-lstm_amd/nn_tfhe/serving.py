@@ -95,8 +95,16 @@ class SlotBootstrapper:
         self.n_lut = n_lut
         self.seq_slots = None
 
-    def set_slots(self, seq_slots: torch.Tensor):
-        self.seq_slots = seq_slots.to(device=self.device, dtype=torch.int32)
+    def set_slots(self, seq_slots):
+        """Slots of the next step's sequences (a list or a tensor).  Host
+        values go through a new pinned buffer with an asynchronous copy, so
+        the host does not wait for the device's queued work (the caching host
+        allocator keeps the buffer until the copy has run)."""
+        if isinstance(seq_slots, torch.Tensor) and seq_slots.is_cuda:
+            self.seq_slots = seq_slots.to(dtype=torch.int32)
+            return
+        host = torch.as_tensor(seq_slots, dtype=torch.int32).pin_memory()
+        self.seq_slots = host.to(self.device, non_blocking=True)
 
     def _per(self, count: int) -> torch.Tensor:
         S = self.seq_slots.numel()
@@ -248,12 +256,12 @@ class EncryptedLSTMServer:
         if new:
             # first timestep of these requests: the forget-gate LUT of their
             # (zero) cell state, under each client's keys
-            self.bs.set_slots(torch.as_tensor([act[i].slot for i in new], dtype=torch.int32))
-            Ln = self.lstm.forget_lut(C[new])
+            self.bs.set_slots([act[i].slot for i in new])
+            Ln = self.lstm.forget_lut(torch.stack([act[i].C for i in new]))
             for j, i in enumerate(new):
                 act[i].L = Ln[j]
         L = torch.stack([r.L for r in act])
-        self.bs.set_slots(torch.as_tensor([r.slot for r in act], dtype=torch.int32))
+        self.bs.set_slots([r.slot for r in act])
         # the step also prepares each request's next forget-gate LUT
         Hn, Cn, Ln = self.lstm.step_pipelined(H, C, X, L)
         self.ticks += 1
