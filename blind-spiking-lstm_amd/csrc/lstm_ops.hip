@@ -527,6 +527,49 @@ __global__ __launch_bounds__(N / 8) void poly_fft_kernel(const int64_t *__restri
     }
 }
 
+// Inverse of poly_fft_kernel: spectra in the reference bin order
+// (F_j = p(zeta^(2j+1)), j < N/2) back to the coefficients, rounded to the
+// nearest integer (jax_inversefourier's rounding, polynomial_jax.py:17-21).
+// Two polynomials per workgroup, as the forward kernel.
+template <int N>
+__global__ __launch_bounds__(N / 8) void poly_ifft_kernel(const cplx *__restrict__ spectra,
+                                                          int64_t cnt, int64_t *__restrict__ out)
+{
+    constexpr int M = Fold<N>::M, T = Fold<N>::T;
+    __shared__ cplx buf[2 * M];
+    __shared__ cplx ztab[N];
+    const int t = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * 2;
+    fill_root_table<N>(ztab);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const bool ok = p0 + c < cnt;
+        const cplx *src = spectra + (p0 + c) * M;
+#pragma unroll
+        for (int pp = 0; pp < 2; pp++) {
+            const int rb = t + pp * T;
+            buf[c * M + phys<N>(rb)] = ok ? src[2 * rb] : cplx{0.0, 0.0};
+            buf[c * M + phys<N>(M - 1 - rb)] = ok ? cconj(src[2 * rb + 1]) : cplx{0.0, 0.0};
+        }
+    }
+    __syncthreads();
+    cplx w[2][4];
+    inverse_any<N>(w, buf, ztab, t);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        if (p0 + c >= cnt)
+            break;
+        int64_t *dst = out + (p0 + c) * N;
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            double lo, hi;
+            unfold_point<N>(w[c][rr], t, rr, ztab, lo, hi);
+            dst[t + rr * T] = (int64_t)rint(lo);
+            dst[t + (rr + 4) * T] = (int64_t)rint(hi);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Packing of n_lwes LWEs (masks of length n_in) into one RLWE
 // (packing, text_jax.py:211-239):
@@ -921,6 +964,28 @@ extern "C" int bslstm_poly_fft(void *stream, const int64_t *polys, int64_t count
                                                     dim3(NN / 8), 0, (hipStream_t)stream, polys,
                                                     count, o));
     return launch_status("polynomial transform");
+}
+
+extern "C" int bslstm_poly_ifft(void *stream, const double *spectra, int64_t count, int64_t degree,
+                                int64_t *out)
+{
+    clear_error();
+    if (!degree_ok(degree))
+        return fail(BSLSTM_INVALID_ARGUMENT, "Unsupported degree");
+    if (count < 0)
+        return fail(BSLSTM_INVALID_ARGUMENT, "bad count");
+    if (count == 0)
+        return BSLSTM_OK;
+    if (!spectra || !out)
+        return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    const int64_t blocks = (count + 1) / 2;
+    if (blocks > 0x7fffffffLL)
+        return fail(BSLSTM_INVALID_ARGUMENT, "count too large");
+    const cplx *sp = reinterpret_cast<const cplx *>(spectra);
+    BSLSTM_DEGREE_SWITCH(degree, hipLaunchKernelGGL((poly_ifft_kernel<NN>), dim3((unsigned)blocks),
+                                                    dim3(NN / 8), 0, (hipStream_t)stream, sp,
+                                                    count, out));
+    return launch_status("inverse polynomial transform");
 }
 
 // Row blocks of the packing MAC: fixed by n_in alone, never by the number of

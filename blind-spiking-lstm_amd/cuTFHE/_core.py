@@ -49,6 +49,7 @@ SIGNATURES = {
                                 ctypes.c_int),
     "bslstm_lut_product_sum": ([_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P], ctypes.c_int),
     "bslstm_poly_fft": ([_P, _P, _I64, _I64, _P], ctypes.c_int),
+    "bslstm_poly_ifft": ([_P, _P, _I64, _I64, _P], ctypes.c_int),
     "bslstm_packing_workspace_bytes": ([_I64, _I64, _I64], ctypes.c_int64),
     "bslstm_packing": ([_P, _P, _P, _I64, _I64, _I64, _P, _I64, _I64, _I64, _I64, _P, _I64, _P],
                        ctypes.c_int),

@@ -126,8 +126,36 @@ def _bucket(batch: int) -> int:
 
 
 def tuned_forms() -> dict:
-    """The autotuner's choices so far: {(degree, l, n_iter, m, bucket): form}."""
+    """The autotuner's choices so far: {(degree, l, n_iter, m, bucket, slots,
+    candidates): form}."""
     return dict(_TUNED)
+
+
+def save_tuning(path: str) -> None:
+    """Write the autotuner's table to a JSON file (one record per shape)."""
+    import json
+    recs = [{"key": [k[0], k[1], k[2], k[3], k[4], bool(k[5]), list(k[6])], "form": f}
+            for k, f in _TUNED.items()]
+    with open(path, "w") as fh:
+        json.dump({"format": "bslstm-tuning", "version": 1, "records": recs}, fh)
+
+
+def load_tuning(path: str) -> int:
+    """Merge a table written by save_tuning; returns the number of records.
+    Records naming an unknown form are skipped."""
+    import json
+    with open(path) as fh:
+        d = json.load(fh)
+    if d.get("format") != "bslstm-tuning" or d.get("version") != 1:
+        raise ValueError(f"{path} is not a version-1 tuning table")
+    n = 0
+    for r in d["records"]:
+        k = r["key"]
+        if r["form"] in FORMS:
+            _TUNED[(int(k[0]), int(k[1]), int(k[2]), int(k[3]), int(k[4]), bool(k[5]),
+                    tuple(k[6]))] = r["form"]
+            n += 1
+    return n
 
 
 def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft: torch.Tensor | None,
@@ -143,9 +171,9 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     thread groups (the latency form for small batches);
     "split": exact integer products as int8 MFMA GEMMs (log_q <= 32; the key
     is prepared once per key tensor and cached), equal to the exact product;
-    "auto": the autotuner's choice for this shape (timed on first use; inside
-    a HIP-graph capture an untuned shape takes "batched").  All forms give
-    bit-identical results.
+    "auto": the autotuner's choice for this shape (timed on first use; an
+    untuned shape inside a HIP-graph capture is an error: capture after one
+    eager call, or after load_tuning).  All forms give bit-identical results.
 
     slots (int32, one per ciphertext): bsk_reshaped is then a pool
     (C, n_iter, 2, l, 2, m, N/2) of client keys and ciphertext i uses
@@ -159,8 +187,14 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
                tuple(candidates))
         if key in _TUNED:
             form = _TUNED[key]
-        elif torch.cuda.is_current_stream_capturing() or batch == 0:
-            form = "batched" if "batched" in candidates else candidates[0]
+        elif batch == 0:
+            form = candidates[0]
+        elif torch.cuda.is_current_stream_capturing():
+            # timing needs a synchronisation, which a capture cannot hold:
+            # never record an untuned guess into a graph
+            raise _core.TFHEInvalidArgument(
+                f"form='auto' for an untuned shape {key[:5]} inside a graph capture; run the "
+                f"same call once eagerly (or load_tuning) before capturing")
         else:
             times = {}
             out = None

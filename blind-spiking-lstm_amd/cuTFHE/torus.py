@@ -41,6 +41,19 @@ def poly_fft(polys: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def poly_ifft(spectra: torch.Tensor) -> torch.Tensor:
+    """Inverse of poly_fft (jax_inversefourier, polynomial_jax.py:17-21):
+    (..., N/2) complex128 -> (..., N) int64, rounded to the nearest integer."""
+    _core.require_device(spectra)
+    sp = spectra.to(torch.complex128).contiguous()
+    N = 2 * sp.shape[-1]
+    cnt = sp.numel() // (N // 2) if N else 0
+    out = torch.empty(*sp.shape[:-1], N, dtype=torch.int64, device=sp.device)
+    _core.check(_lib().bslstm_poly_ifft(_core.stream_of(sp), sp.data_ptr(), cnt, N,
+                                        out.data_ptr()))
+    return out
+
+
 def linear_extract(ct: torch.Tensor, weights: torch.Tensor, bias: torch.Tensor | None,
                    log_q: int, support: torch.Tensor | None = None, check_support: bool = True):
     """Homomorphic Linear + coefficient-0 extraction (Spike_LSTM.py:46-61).

@@ -101,18 +101,17 @@ def _uniform64(shape, gen, device):
 
 def _negacyclic_mul_key(a, s):
     """Exact a(X)*s(X) mod (X^N+1, 2^64) for int64 a (..., N) and a binary
-    key s (N,) on the device: four 16-bit limbs of a, each product rounded
-    from an FP64 transform (|coefficients| < 2^16 * N, exact)."""
-    import torch
-    N = a.shape[-1]
-    k = torch.arange(N, device=a.device, dtype=torch.float64)
-    tw = torch.polar(torch.ones_like(k), -torch.pi * k / N)
-    fs = torch.fft.fft(s.to(torch.float64) * tw)
-    acc = torch.zeros_like(a)
+    key s (N,) on the device: four 16-bit limbs of a, each product through
+    the engine's negacyclic transform pair (bslstm_poly_fft / _poly_ifft)
+    and rounded (|coefficients| < 2^16 * N, exact)."""
+    from cuTFHE import torus
+    fs = torus.poly_fft(s.to(a.dtype))
+    acc = None
     for limb in range(4):
-        part = ((a >> (16 * limb)) & 0xFFFF).to(torch.float64)
-        c = torch.fft.ifft(torch.fft.fft(part * tw) * fs) / tw
-        acc = acc + torch.round(c.real).to(torch.int64) * _wrap_const(1 << (16 * limb))
+        part = (a >> (16 * limb)) & 0xFFFF
+        c = torus.poly_ifft(torus.poly_fft(part) * fs)
+        term = c * _wrap_const(1 << (16 * limb))
+        acc = term if acc is None else acc + term
     return acc
 
 

@@ -270,6 +270,12 @@ int bslstm_lut_product_sum(void *stream, const int64_t *rlwe, const double *lut_
 int bslstm_poly_fft(void *stream, const int64_t *polys, int64_t count, int64_t degree,
                     double *out);
 
+/* Inverse of bslstm_poly_fft: spectra complex128 (count, N/2) in the same
+ * bin order -> int64 (count, N), each coefficient rounded to the nearest
+ * integer (jax_inversefourier, schemas/polynomial_jax.py:17-21). */
+int bslstm_poly_ifft(void *stream, const double *spectra, int64_t count, int64_t degree,
+                     int64_t *out);
+
 /* Workspace of bslstm_packing in bytes (-1 on bad arguments). */
 int64_t bslstm_packing_workspace_bytes(int64_t packs, int64_t n_in, int64_t degree);
 

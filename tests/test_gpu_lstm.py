@@ -492,3 +492,19 @@ def test_serving_results_same_alone_and_in_the_mix(cuda_device, tmp_path):
         ph = _phase((mixed[i][1][0], mixed[i][1][1]), secrets[specs[i][0]][0], m.dp)[:m.hidden]
         frac = np.abs(ph / 2.0 ** 56 - np.rint(ph / 2.0 ** 56))
         assert frac.max() < 0.05 and np.abs(np.rint(ph / 2.0 ** 56)).max() <= specs[i][1]
+
+
+@pytest.mark.parametrize("N", [256, 1024, 2048])
+def test_poly_ifft_inverts_poly_fft(cuda_device, T, N):
+    """bslstm_poly_ifft(bslstm_poly_fft(p)) = p for integer polynomials whose
+    transform is exact in FP64 (|p| < 2^40), and the device key generation's
+    exact product through the pair equals the int128 schoolbook mod 2^64."""
+    import setup_crypto as SC
+    rng = np.random.default_rng(N)
+    p = rng.integers(-2 ** 40, 2 ** 40, (5, N))
+    np.testing.assert_array_equal(host(T.poly_ifft(T.poly_fft(dev(p)))), p)
+    a = rng.integers(-2 ** 63, 2 ** 63, (3, N), dtype=np.int64)
+    s = rng.integers(0, 2, N)
+    got = host(SC._negacyclic_mul_key(dev(a), dev(s)))
+    want = np.stack([X.negacyclic_mul(a[i], s, 64) for i in range(3)])
+    np.testing.assert_array_equal(got, want)

@@ -91,3 +91,53 @@ def test_all_binary_vectors_product_matches_oracle():
     from cuTFHE.utils import all_binary_vectors
     for c in range(1, 5):
         np.testing.assert_array_equal(all_binary_vectors(c), O.all_binary_vectors(c))
+
+
+def test_tuning_table_round_trip(tmp_path):
+    from cuTFHE import multiply
+    saved = dict(multiply._TUNED)
+    try:
+        multiply._TUNED.clear()
+        multiply._TUNED[(2048, 2, 234, 8, 6, False, ("plain", "batched", "rows"))] = "rows"
+        multiply._TUNED[(2048, 3, 250, 8, 14, True, ("plain", "rows"))] = "plain"
+        path = str(tmp_path / "tune.json")
+        multiply.save_tuning(path)
+        want = dict(multiply._TUNED)
+        multiply._TUNED.clear()
+        assert multiply.load_tuning(path) == 2
+        assert multiply._TUNED == want
+    finally:
+        multiply._TUNED.clear()
+        multiply._TUNED.update(saved)
+
+
+def test_plaintext_step_lut_negacyclic_extension():
+    """The plaintext gate equals the test polynomial get_plain_LUT builds,
+    read the way the blind rotation reads it (coefficient 0 of X^-phi * T,
+    negacyclic), over the whole torus and not only [-s, s)."""
+    from nn_tfhe.plaintext import lut_margin, step_lut
+    from nn_tfhe.utils_spike import get_plain_LUT
+    N, n_lut, s, gamma = 256, 4, 4.0, 0.7
+    for k in range(n_lut):
+        T = np.asarray(get_plain_LUT(N, n_lut, k, s, gamma), np.float64)
+        for j in range(2 * N // n_lut):                # LUT entries over the whole torus
+            phi = j * n_lut                            # rounded phase, multiple of n_lut
+            read = T[phi + k] if phi + k < N else -T[phi + k - N]
+            pre = j * 2 * s / (N // n_lut)             # phase j entries above zero
+            if lut_margin(pre, gamma, s) < 2 * 2 * s / (N // n_lut):
+                continue                               # grid rounding at discontinuities
+            assert read == step_lut(pre, gamma, s), (k, j, pre, read)
+
+
+def test_plaintext_relu_ramp_and_noise_model():
+    from nn_tfhe.plaintext import lut_index_noise, relu_ramp
+    import parameters_crypto as PC
+    p = np.array([-7.0, -5.0, -4.5, -1.0, 0.0, 0.5, 3.99, 4.5, 7.9])
+    got = relu_ramp(p, 4.0, 1024)
+    np.testing.assert_allclose(got, [-1.0, -3.0, -3.5, 0.0, 0.0, 0.5, 3.98828125, 0.0, 0.0])
+    # reference parameters: modulus switch (235 uniform terms) and key switch
+    # (sqrt(2048*20) * 1 * 2^48.6 / 2^54 entries at n_lut 4)
+    sig = lut_index_noise(PC.dict_params_lut, PC.dict_params_ks_LWE, PC.degree, PC.q, 4)
+    ms = np.sqrt(235 / 12)
+    ks = np.sqrt(2048 * 20) * 2.0 ** 48.6 / 2.0 ** 54
+    assert abs(sig - np.sqrt(ms ** 2 + ks ** 2)) < 1e-9
