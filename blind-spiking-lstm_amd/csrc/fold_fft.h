@@ -9,8 +9,9 @@
 //  * forward_pair / inverse_pair: Stockham radix-4 (+ one radix-2 pass when
 //    log2(N/2) is odd) through LDS, natural bin order, any N in 128..4096.
 //  * forward_wave / inverse_wave (N in 512..2048): one radix-W cross-wave
-//    stage, then four in-wave radix-4 passes over 256-point regions with no
-//    workgroup barrier; bin b lives at phys(b).
+//    stage through LDS, then four in-wave radix-4 passes over 256-point
+//    regions whose data move between passes by cross-lane exchanges
+//    (v_permlane32/16_swap, DPP), not LDS; bin b lives at phys(b).
 #pragma once
 
 #include "tfhe_device.h"
@@ -174,8 +175,8 @@ __device__ __forceinline__ void inverse_pair(cplx (&v)[NP][4], cplx *buf, const 
 // Wave-local transform for N in {512, 1024, 2048}: W = N/512 waves per
 // ciphertext, each wave owning a 256-point region of the N/2-point transform.
 // One radix-W stage crosses waves (a register butterfly plus one LDS
-// exchange); the remaining four radix-4 Stockham passes run inside a wave,
-// ordered by the wave's own in-order LDS traffic, with no workgroup barrier.
+// exchange); the remaining four radix-4 passes run inside a wave with
+// cross-lane register exchanges between them and no workgroup barrier.
 // Bin b of the result is element b/W of region b mod W (see region_slot).
 // ---------------------------------------------------------------------------
 template <int N>
@@ -185,8 +186,9 @@ struct Wave {
 };
 
 // Position of element e of region k in LDS.  The XOR keeps every access
-// pattern of the transform (contiguous reads, the strided Stockham writes of
-// each pass) and the MAC's gathers across regions free of bank conflicts.
+// pattern of the transform (contiguous reads, the digit-reversed output
+// writes of the in-wave passes) and the MAC's gathers across regions free of
+// bank conflicts.
 template <int N>
 __device__ __forceinline__ int region_slot(int k, int e)
 {
@@ -205,53 +207,189 @@ __device__ __forceinline__ int phys(int b)
 
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
-// Four in-wave radix-4 Stockham passes over the 256-point regions reg[p]
-// (lane l holds points l + 64r).  If FROM_REGS the first pass takes v[p][r]
-// instead of reading LDS.  If TO_REGS (inverse only) the last pass leaves
-// its outputs (points l + 64r) in v instead of LDS.
+// ---------------------------------------------------------------------------
+// Cross-lane exchanges.  xswap<S>(a, b) exchanges lane bit log2(S) with the
+// register pair (a, b): a lane with the bit clear ends with (its a, partner's
+// a), a lane with it set with (partner's b, its b), the partner being lane
+// l ^ S.  Four such swaps transpose a 4x4 block of (lane digit, register
+// index), which moves the next radix-4 digit of the transform from the lanes
+// into the registers without touching LDS.
+//  S = 32, 16: v_permlane32_swap / v_permlane16_swap (gfx950), one
+//              instruction per 32-bit word pair.
+//  S = 8, 4:   two DPP moves with bank masks (the lanes of one half of each
+//              row take the partner's word, the others keep their own).
+//  S = 2, 1:   a quad_perm DPP move of the outgoing word and two selects.
+// ---------------------------------------------------------------------------
+template <int S>
+__device__ __forceinline__ void xswap_u32(uint32_t &a, uint32_t &b, bool hi)
+{
+    if constexpr (S == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+        a = r[0];
+        b = r[1];
+    } else if constexpr (S == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+        a = r[0];
+        b = r[1];
+    } else if constexpr (S == 8 || S == 4) {
+        // DPP row_ror:n -- lane i of a row reads lane (i - n) mod 16.  Lanes
+        // with the bit set (banks 2,3 for S = 8; banks 1,3 for S = 4) read
+        // lane i - S; the others read lane i + S (row_ror:16-S).
+        constexpr int FROM_LO = 0x120 + S, FROM_HI = 0x120 + 16 - S;
+        constexpr int BM_HI = S == 8 ? 0xC : 0xA, BM_LO = S == 8 ? 0x3 : 0x5;
+        const uint32_t na = (uint32_t)__builtin_amdgcn_update_dpp((int)a, (int)b, FROM_LO, 0xF, BM_HI, false);
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)b, (int)a, FROM_HI, 0xF, BM_LO, false);
+        a = na;
+        b = nb;
+    } else {
+        static_assert(S == 2 || S == 1, "lane strides 1..32");
+        constexpr int QP = S == 1 ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
+        const uint32_t out = hi ? a : b;
+        const uint32_t in = (uint32_t)__builtin_amdgcn_mov_dpp((int)out, QP, 0xF, 0xF, true);
+        a = hi ? in : a;
+        b = hi ? b : in;
+    }
+}
+
+__device__ __forceinline__ uint32_t lo32(double d) { return (uint32_t)__double2loint(d); }
+__device__ __forceinline__ uint32_t hi32(double d) { return (uint32_t)__double2hiint(d); }
+__device__ __forceinline__ double mk64(uint32_t lo, uint32_t hi) { return __hiloint2double((int)hi, (int)lo); }
+
+template <int S>
+__device__ __forceinline__ void xswap(cplx &a, cplx &b, int l)
+{
+    const bool hi = (l & S) != 0;
+    uint32_t a0 = lo32(a.re), a1 = hi32(a.re), a2 = lo32(a.im), a3 = hi32(a.im);
+    uint32_t b0 = lo32(b.re), b1 = hi32(b.re), b2 = lo32(b.im), b3 = hi32(b.im);
+    xswap_u32<S>(a0, b0, hi);
+    xswap_u32<S>(a1, b1, hi);
+    xswap_u32<S>(a2, b2, hi);
+    xswap_u32<S>(a3, b3, hi);
+    a = {mk64(a0, a1), mk64(a2, a3)};
+    b = {mk64(b0, b1), mk64(b2, b3)};
+}
+
+// Transpose of the register index (2 bits) with the lane digit at lane
+// strides (SH, SL): afterwards the lanes hold the former register digit.
+template <int SH, int SL>
+__device__ __forceinline__ void xtranspose(cplx (&x)[4], int l)
+{
+    xswap<SH>(x[0], x[2], l);
+    xswap<SH>(x[1], x[3], l);
+    xswap<SL>(x[0], x[1], l);
+    xswap<SL>(x[2], x[3], l);
+}
+
+// Element of the 256-point region that holds transform output k0 + 4k1 +
+// 16k2 + 64k3 after the in-wave passes: lane 16k0 + 4k1 + k2, register k3.
+__device__ __forceinline__ int wave_bin(int l, int r)
+{
+    return (l >> 4) + 4 * ((l >> 2) & 3) + 16 * (l & 3) + 64 * r;
+}
+
+// In-wave 256-point transform of the regions reg[p]: four radix-4 passes,
+// the data staying in registers between them.  Forward (decimation in
+// frequency): lane l, register r start with point l + 64r; each pass is a
+// register butterfly over the register digit followed by its twiddle, then a
+// cross-lane transpose brings the next input digit into the registers; the
+// outputs end at lane 16k0 + 4k1 + k2, register k3 (bin k0 + 4k1 + 16k2 +
+// 64k3).  Inverse (decimation in time, unnormalised): the same steps in
+// reverse order with conjugate twiddles.  If FROM_REGS the input is v[p][r]
+// (forward) instead of LDS; if TO_REGS (inverse) the outputs (points l + 64r)
+// stay in v instead of LDS.  LDS layout: element e of region rg of
+// polynomial p at polys[p*M + region_slot(rg, e)].
 template <int N, bool INV, bool FROM_REGS, bool TO_REGS, bool FULL = false, int NP = 2>
 __device__ __forceinline__ void wave_passes(cplx (&v)[NP][4], cplx *polys, int rg, int l,
                                             const cplx *ztab)
 {
-    // element e of this wave's region rg of polynomial p:
-    // polys[p*M + region_slot(rg, e)]
     constexpr int M = N / 2;
+    constexpr int E256 = M / 64;                 // zeta exponent of exp(-2 pi i/256)
+    cplx x[NP][4];
 #pragma unroll
-    for (int Ns = 1; Ns < 256; Ns *= 4) {
-        cplx x[NP][4];
-        if (FROM_REGS && Ns == 1) {
+    for (int p = 0; p < NP; p++)
 #pragma unroll
-            for (int p = 0; p < NP; p++)
+        for (int r = 0; r < 4; r++)
+            x[p][r] = FROM_REGS ? v[p][r]
+                                : polys[p * M + region_slot<N>(rg, INV ? wave_bin(l, r) : l + 64 * r)];
+    // twiddle exponents of the three twiddled passes (units of exp(-2 pi i/256)):
+    // pass 1 w256^(l k), pass 2 w64^((l & 15) k), pass 3 w16^((l & 3) k)
+    const int tw[3] = {l, 4 * (l & 15), 16 * (l & 3)};
+    auto twiddle = [&](int s) {
+        cplx w[3];
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    x[p][r] = v[p][r];
-        } else {
-#pragma unroll
-            for (int p = 0; p < NP; p++)
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    x[p][r] = polys[p * M + region_slot<N>(rg, l + 64 * r)];
-        }
-        const int k = l & (Ns - 1);
+        for (int k = 1; k < 4; k++)
+            w[k - 1] = zeta_pow<N, FULL>(ztab, (INV ? -E256 : E256) * tw[s] * k);
 #pragma unroll
         for (int p = 0; p < NP; p++)
-            r4_twiddle<N, FULL>(x[p], k, Ns, INV, ztab);
-        if (TO_REGS && Ns == 64) {
+#pragma unroll
+            for (int k = 1; k < 4; k++)
+                x[p][k] = cmul(x[p][k], w[k - 1]);
+    };
+    if constexpr (!INV) {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            dft4(x[p], false);
+        }
+        twiddle(0);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            xtranspose<32, 16>(x[p], l);
+            dft4(x[p], false);
+        }
+        twiddle(1);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            xtranspose<8, 4>(x[p], l);
+            dft4(x[p], false);
+        }
+        twiddle(2);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            xtranspose<2, 1>(x[p], l);
+            dft4(x[p], false);
+        }
+        wave_sync();
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                polys[p * M + region_slot<N>(rg, wave_bin(l, r))] = x[p][r];
+    } else {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            dft4(x[p], true);
+            xtranspose<2, 1>(x[p], l);
+        }
+        twiddle(2);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            dft4(x[p], true);
+            xtranspose<8, 4>(x[p], l);
+        }
+        twiddle(1);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            dft4(x[p], true);
+            xtranspose<32, 16>(x[p], l);
+        }
+        twiddle(0);
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+            dft4(x[p], true);
+        if constexpr (TO_REGS) {
 #pragma unroll
             for (int p = 0; p < NP; p++)
 #pragma unroll
                 for (int r = 0; r < 4; r++)
                     v[p][r] = x[p][r];
-            return;
+        } else {
+            wave_sync();
+#pragma unroll
+            for (int p = 0; p < NP; p++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    polys[p * M + region_slot<N>(rg, l + 64 * r)] = x[p][r];
         }
-        wave_sync();
-        const int idxD = (l / Ns) * 4 * Ns + k;
-#pragma unroll
-        for (int p = 0; p < NP; p++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                polys[p * M + region_slot<N>(rg, idxD + r * Ns)] = x[p][r];
-        wave_sync();
     }
 }
 
