@@ -70,16 +70,18 @@ def GLEV_polynomial(x, beta: float, l: int, q: float) -> np.ndarray:
     return np.asarray(x, np.float64) * basis
 
 
-def coef_rotation(polynomial: torch.Tensor, alpha: torch.Tensor) -> torch.Tensor:
+def coef_rotation(polynomial: torch.Tensor, alpha) -> torch.Tensor:
     """polynomial * X^alpha mod X^N+1 (polynomial_jax.py:150-169), batched
-    over the leading axis with one integer exponent per row."""
+    over the leading axis with one integer exponent per row (or one for all),
+    on the HIP rotation kernel (bslstm_rotate_poly).  int64 or integer-valued
+    float64 (exact below 2^53); the result keeps the input's dtype and is not
+    reduced mod q, as the reference's rotation."""
+    from cuTFHE import torus
     N = polynomial.shape[-1]
-    alpha = alpha.to(torch.int64).reshape(-1, 1)
-    j = torch.arange(N, device=polynomial.device).reshape(1, -1)
-    w = torch.remainder(j - alpha, 2 * N)
-    src = torch.where(w < N, w, w - N)
-    sign = torch.where(w < N, 1, -1).to(polynomial.dtype)
-    return torch.gather(polynomial.reshape(-1, N), 1, src.expand(polynomial.reshape(-1, N).shape[0], N)) * sign
+    if isinstance(alpha, torch.Tensor) and alpha.dtype != torch.int64:
+        alpha = torch.round(alpha.to(torch.float64)).to(torch.int64)
+    out = torus.rotate_poly(polynomial.reshape(-1, N), alpha, 64).reshape(polynomial.shape)
+    return out if polynomial.dtype == torch.int64 else out.to(polynomial.dtype)
 
 
 # ---------------------------------------------------------------------------

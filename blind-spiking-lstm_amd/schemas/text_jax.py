@@ -1,6 +1,9 @@
 """RLWE text operations of the reference's schemas/text_jax.py that sit on the
-bootstrapping path, as torch operations on the GPU.  (The hot many-LUT tail
-of the bootstrap uses the fused HIP kernel in schemas/RLWE_jax.py instead.)
+bootstrapping path, on the GPU through the HIP kernels of cuTFHE.torus.
+float64 tensors (the reference's carrier of torus values, integer-valued and
+exact below 2^53) go through the int64 kernels and come back as float64.
+(The hot many-LUT tail of the bootstrap uses the fused HIP kernel in
+schemas/RLWE_jax.py.)
 """
 from __future__ import annotations
 
@@ -9,35 +12,42 @@ import math
 import torch
 
 from cuTFHE.utils import all_binary_vectors  # noqa: F401  (text_jax.py:440-454 twin)
-from schemas.polynomial_jax import coef_rotation
+
+
+def _as_carrier(x: torch.Tensor, like: torch.Tensor) -> torch.Tensor:
+    """int64 kernel result back in the caller's dtype (float64 stays float64)."""
+    return x if like.dtype == torch.int64 else x.to(like.dtype)
 
 
 def sample_extract(ciphertext, alpha: int):
     """text_jax.py:191-208: LWE of coefficient alpha of an RLWE (A, B),
     batched over leading axes.  a'_j = A[alpha-j] (j <= alpha), else
-    -A[alpha-j+N]; b' = B[alpha] (shape (..., 1))."""
+    -A[alpha-j+N]; b' = B[alpha] (shape (..., 1)).  Runs as X^(-alpha)
+    rotation + extraction of coefficient 0 (the same LWE) on the HIP
+    rotate/extract kernels; no reduction mod q (log q = 64 wraps nothing
+    below 2^63)."""
+    from cuTFHE import torus
     A, B = ciphertext[0], ciphertext[1]
-    N = A.shape[-1]
-    j = torch.arange(N, device=A.device)
-    src = torch.where(j <= alpha, alpha - j, alpha - j + N)
-    sign = torch.where(j <= alpha, 1, -1).to(A.dtype)
-    return A[..., src] * sign, B[..., alpha:alpha + 1]
+    lead, N = A.shape[:-1], A.shape[-1]
+    ct = torch.stack([torus._i64(A).reshape(-1, N), torus._i64(B).reshape(-1, N)], dim=1)
+    ct = torus.rotate_poly(ct, -int(alpha), 64)
+    masks, bodies = torus.rotate_extract(ct, None, 64, 1)
+    return (_as_carrier(masks[:, 0].reshape(*lead, N), A),
+            _as_carrier(bodies.reshape(*lead, 1), B))
 
 
 def rotate_ciphertext(ciphertext, exponent, log_q: int | None = None):
     """text_jax.py:348-362: multiply both rows by X^exponent (one exponent per
-    leading row).  int64 ciphertexts go through the HIP rotation kernel
-    (log_q defaults to 64: the result is the same class mod any q)."""
+    leading row) on the HIP rotation kernel.  log_q defaults to 64: int64
+    classes are the same mod any q, and float64 values (|x| < 2^53) come back
+    unreduced, as the reference's float rotation leaves them."""
+    from cuTFHE import torus
     A, B = ciphertext[0], ciphertext[1]
-    if A.dtype == torch.int64:
-        from cuTFHE import torus
-        lq = 64 if log_q is None else log_q
-        return torus.rotate_poly(A, exponent, lq), torus.rotate_poly(B, exponent, lq)
-    lead = A.shape[:-1]
-    e = torch.as_tensor(exponent, device=A.device).reshape(-1)
-    rows = A.reshape(-1, A.shape[-1]).shape[0]
-    e = e.expand(rows) if e.numel() == 1 else e
-    return (coef_rotation(A, e).reshape(*lead, -1), coef_rotation(B, e).reshape(*lead, -1))
+    lq = 64 if log_q is None else log_q
+    e = exponent
+    if isinstance(e, torch.Tensor) and e.dtype != torch.int64:
+        e = torch.round(e.to(torch.float64)).to(torch.int64)
+    return (_as_carrier(torus.rotate_poly(A, e, lq), A), _as_carrier(torus.rotate_poly(B, e, lq), B))
 
 
 # ---------------------------------------------------------------------------

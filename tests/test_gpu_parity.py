@@ -311,3 +311,94 @@ def test_heaviside_bootstrapper_cuboot(cuda_device):
                               [dev(lut[:, 0], torch.float64), dev(lut[:, 1], torch.float64)])
     dec, _ = decrypt_outputs(case, host(masks), host(bodies)[..., 0])
     np.testing.assert_array_equal(dec, expected_mod(case))
+
+
+def test_br_and_bootstrapping_vs_exact(cuda_device):
+    """BR() (RLWE_jax.py:305-315: blind rotation, then X^(-b_ms)) against the
+    int128 oracle, and bootstrapping() (RLWE_jax.py:233-257), batched and on
+    one ciphertext, decrypting to the identity LUT."""
+    from schemas.RLWE_jax import BR, bootstrapping
+    case = make_case(N=512, n=48, L=3, log_beta=7, B=12, msg_bits=3, seed=41, sigma_bsk=2.0 ** 5)
+    B, N, q = 12, case.N, 2.0 ** 32
+    lut = np.broadcast_to(case.lut, (B, 2, N))
+    pk, b = dev(case.a, torch.float64), dev(case.b[:, None], torch.float64)
+    lut_rows = [dev(lut[:, 0], torch.float64), dev(lut[:, 1], torch.float64)]
+    bsk = dev(case.bsk_ordered)
+    br = host(BR((pk, b), lut_rows, bsk, q, 2.0 ** 7, 3, N, 3, None, 1))
+    assert br.shape == (B, 2, N) and br.dtype == np.float64
+    idx = O.rotation_indices(case.a, 32, N, 3, 1)
+    exact = X.blind_rotate(case.lut, case.bsk_coef_k, idx, 7, 32, 3)
+    b_ms = np.rint(2 * N * case.b.astype(np.float64) / q).astype(np.int64)
+    for i in range(B):
+        for c in range(2):
+            want = O.coef_rotation(exact[i, c], -b_ms[i])
+            assert np.abs(O.centered_mod(br[i, c].astype(np.int64) - want, 32)).max() <= BR_COEF_TOL
+    masks, bodies = bootstrapping((pk, b), lut_rows, bsk, q, 2.0 ** 7, 3, N, 3)
+    masks, bodies = host(masks), host(bodies)
+    assert masks.shape == (B, N) and bodies.shape == (B, 1)
+    dec, _ = decrypt_outputs(case, masks[:, None], bodies)
+    np.testing.assert_array_equal(dec, expected_mod(case))
+    # bootstrapping()'s body is coefficient 0 of BR()'s body row (mod q: the
+    # extraction reduces, BR's float rotation does not)
+    assert (O.centered_mod(bodies[:, 0].astype(np.int64) - br[:, 1, 0].astype(np.int64), 32) == 0).all()
+    m1, b1 = bootstrapping((pk[0], b[0]), [lut_rows[0][0], lut_rows[1][0]], bsk, q, 2.0 ** 7, 3, N, 3)
+    np.testing.assert_array_equal(host(m1), masks[0])
+    np.testing.assert_array_equal(host(b1).reshape(-1), bodies[0])
+
+
+def test_heaviside_bootstrapper_call(cuda_device):
+    """HeavysideBoostrapper.__call__ (activation.py:98-103): key switch N -> n,
+    then bootstrapping() with one LUT; decrypts to LUT(m) and equals cuboot
+    with n_lut = 1."""
+    from nn_tfhe.activation import HeavysideBoostrapper
+    case = make_case(N=512, n=48, L=3, log_beta=7, B=16, msg_bits=3, seed=43, sigma_bsk=2.0 ** 5)
+    rng = np.random.default_rng(43)
+    a_big, b_big = O.lwe_encrypt(rng, case.msg * case.delta_in, case.S, 32, 2.0 ** 12)
+    ka, kb = O.keyswitch_key(rng, case.s, case.S, 32, 4, 6, 2.0 ** 8)
+    p_lut = {"q": 2.0 ** 32, "beta_bs": 2 ** 7, "l_bs": 3, "degree": 512}
+    p_ks = {"q": 2.0 ** 32, "beta_ks": 16, "l_ks": 6, "degree": 48}
+    bs = HeavysideBoostrapper(torch.as_tensor(case.bsk_ordered), (ka.astype(np.float64),
+                              kb[..., None].astype(np.float64)), None, None, {}, p_lut, p_ks, {},
+                              3, None, 1)
+    lut = np.broadcast_to(case.lut, (16, 2, 512))
+    h = (dev(a_big, torch.float64), dev(b_big[:, None], torch.float64))
+    lut_rows = [dev(lut[:, 0], torch.float64), dev(lut[:, 1], torch.float64)]
+    masks, bodies = bs(h, lut_rows)
+    masks, bodies = host(masks), host(bodies)
+    dec, _ = decrypt_outputs(case, masks[:, None], bodies)
+    np.testing.assert_array_equal(dec, expected_mod(case))
+    cm, cb = bs.cuboot(h, lut_rows)
+    np.testing.assert_array_equal(host(cm)[:, 0], masks)
+    np.testing.assert_array_equal(host(cb)[:, 0, 0], bodies[:, 0])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.int64])
+def test_text_rotation_and_extraction_on_hip_kernels(cuda_device, dtype):
+    """text_jax.sample_extract / rotate_ciphertext and polynomial_jax.coef_rotation
+    (text_jax.py:191-208,348-362; polynomial_jax.py:150-169) through the HIP
+    kernels, float64 and int64 carriers, against the oracle."""
+    from schemas.polynomial_jax import coef_rotation
+    from schemas.text_jax import rotate_ciphertext, sample_extract
+    rng = np.random.default_rng(44)
+    N, R = 256, 6
+    A = rng.integers(-2 ** 31, 2 ** 31, (R, N))
+    Bq = rng.integers(-2 ** 31, 2 ** 31, (R, N))
+    e = rng.integers(-3 * N, 3 * N, R)
+    ta, tb = dev(A, dtype), dev(Bq, dtype)
+    got = host(coef_rotation(ta, dev(e, torch.int64)))
+    assert got.dtype == (np.float64 if dtype == torch.float64 else np.int64)
+    for i in range(R):
+        np.testing.assert_array_equal(got[i].astype(np.int64), O.coef_rotation(A[i], e[i]))
+    ra, rb = rotate_ciphertext((ta, tb), dev(e, torch.int64))
+    ra, rb = host(ra), host(rb)
+    for i in range(R):
+        np.testing.assert_array_equal(ra[i].astype(np.int64), O.coef_rotation(A[i], e[i]))
+        np.testing.assert_array_equal(rb[i].astype(np.int64), O.coef_rotation(Bq[i], e[i]))
+    for alpha in (0, 1, 77, N - 1):
+        ma, mb = sample_extract((ta, tb), alpha)
+        ma, mb = host(ma), host(mb)
+        assert ma.shape == (R, N) and mb.shape == (R, 1)
+        for i in range(R):
+            wa, wb = O.sample_extract((A[i], Bq[i]), alpha)
+            np.testing.assert_array_equal(ma[i].astype(np.int64), np.asarray(wa, np.int64))
+            assert int(mb[i, 0]) == int(np.asarray(wb).reshape(-1)[0])
