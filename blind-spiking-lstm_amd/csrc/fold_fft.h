@@ -218,7 +218,7 @@ __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 //              instruction per 32-bit word pair.
 //  S = 8, 4:   two DPP moves with bank masks (the lanes of one half of each
 //              row take the partner's word, the others keep their own).
-//  S = 2, 1:   a quad_perm DPP move of the outgoing word and two selects.
+//  S = 2, 1:   quad_perm DPP moves of both words and two selects.
 // ---------------------------------------------------------------------------
 template <int S>
 __device__ __forceinline__ void xswap_u32(uint32_t &a, uint32_t &b, bool hi)
@@ -243,11 +243,12 @@ __device__ __forceinline__ void xswap_u32(uint32_t &a, uint32_t &b, bool hi)
         b = nb;
     } else {
         static_assert(S == 2 || S == 1, "lane strides 1..32");
+        // both partner words, then the selects: no select feeds a DPP move
         constexpr int QP = S == 1 ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
-        const uint32_t out = hi ? a : b;
-        const uint32_t in = (uint32_t)__builtin_amdgcn_mov_dpp((int)out, QP, 0xF, 0xF, true);
-        a = hi ? in : a;
-        b = hi ? b : in;
+        const uint32_t pa = (uint32_t)__builtin_amdgcn_mov_dpp((int)a, QP, 0xF, 0xF, true);
+        const uint32_t pb = (uint32_t)__builtin_amdgcn_mov_dpp((int)b, QP, 0xF, 0xF, true);
+        a = hi ? pb : a;
+        b = hi ? b : pa;
     }
 }
 

@@ -97,6 +97,8 @@ class HeavysideBoostrapper:
         self.n_lut = n_lut
 
     def __call__(self, h, encrypted_LUT):
+        """activation.py:120-146: key switch N -> n, then one bootstrap per LWE
+        with the encrypted test polynomial (one LUT)."""
         c_lwe_ks = key_switch_LWE(self.key_switching_key_bs, h, self.dict_params_ks_LWE)
         p = self.dict_params_lut
         return bootstrapping(c_lwe_ks, encrypted_LUT, self.bootstrapping_key, p["q"], p["beta_bs"],

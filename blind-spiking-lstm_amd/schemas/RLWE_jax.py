@@ -253,7 +253,7 @@ def BR(c_lwe_ks, encrypted_LUT, bsk, q, beta, l, degree, collapse, all_rot_possi
 
 def bootstrapping(lwe_ciphertext, LUT, BSK, q, beta, l, degree, collapse, all_rot_fft=None,
                   n_lut: int = 1):
-    """RLWE_jax.py:233-257 (the reference's pure-JAX path) on the GPU path:
+    """RLWE_jax.py:234-274 (the reference's pure-JAX path) on the GPU path:
     same mathematics, rotation applied after the blind rotation."""
     pk, b = lwe_ciphertext
     single = pk.dim() == 1

@@ -315,7 +315,7 @@ def test_heaviside_bootstrapper_cuboot(cuda_device):
 
 def test_br_and_bootstrapping_vs_exact(cuda_device):
     """BR() (RLWE_jax.py:305-315: blind rotation, then X^(-b_ms)) against the
-    int128 oracle, and bootstrapping() (RLWE_jax.py:233-257), batched and on
+    int128 oracle, and bootstrapping() (RLWE_jax.py:234-274), batched and on
     one ciphertext, decrypting to the identity LUT."""
     from schemas.RLWE_jax import BR, bootstrapping
     case = make_case(N=512, n=48, L=3, log_beta=7, B=12, msg_bits=3, seed=41, sigma_bsk=2.0 ** 5)
@@ -347,7 +347,7 @@ def test_br_and_bootstrapping_vs_exact(cuda_device):
 
 
 def test_heaviside_bootstrapper_call(cuda_device):
-    """HeavysideBoostrapper.__call__ (activation.py:98-103): key switch N -> n,
+    """HeavysideBoostrapper.__call__ (activation.py:120-146): key switch N -> n,
     then bootstrapping() with one LUT; decrypts to LUT(m) and equals cuboot
     with n_lut = 1."""
     from nn_tfhe.activation import HeavysideBoostrapper
