@@ -273,9 +273,12 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
         // r: digits source A(X^k); the B row takes B(X^k) now (B <- B + B(X^k)
         // + B_ks, the sum wraps mod 2^64)
         int64_t r[8];
+        int tz[8];
 #pragma unroll
-        for (int i = 0; i < 8; i++)
+        for (int i = 0; i < 8; i++) {
             r[i] = tmod((uint64_t)xb[t + i * T], lq);
+            tz[i] = trailing_zeros((uint64_t)r[i]);
+        }
 #pragma unroll
         for (int c = 0; c < NR; c++)
             if (h * NR + c == 1)
@@ -283,8 +286,8 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
                 for (int i = 0; i < 8; i++)
                     X[c][i] += (uint64_t)xb[N + t + i * T];
         // ---- gadget product with the Galois key of this step; group h
-        // transforms the level pairs h, h + RG, ..., starting the digit
-        // recursion at each pair from residual_at
+        // transforms the level pairs h, h + RG, ..., each pair's digits read
+        // straight from bit fields of the class (digit_at)
         const cplx *key = gkey + (size_t)st * l * 2 * M;
         // digits of level pair P, their transforms, and the products with the
         // key rows of those levels added into A (in level order)
@@ -293,12 +296,10 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
             const int lev = 2 * P;
             double dg[2][8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                int64_t res = residual_at(r[i], lq - lev * lb, lq);
+            for (int i = 0; i < 8; i++)
 #pragma unroll
                 for (int c = 0; c < 2; c++)
-                    dg[c][i] = lev + c < l ? (double)next_digit(res, lq - (lev + c + 1) * lb) : 0.0;
-            }
+                    dg[c][i] = lev + c < l ? (double)digit_at((uint64_t)r[i], lev + c, lb, lq, tz[i]) : 0.0;
 #pragma unroll
             for (int c = 0; c < 2; c++)
 #pragma unroll

@@ -61,6 +61,36 @@ __device__ __forceinline__ int64_t residual_at(int64_t r, int S, int lq)
     return up ? (int64_t)(rem - (1ULL << S)) : (int64_t)rem;
 }
 
+// The same digits without the recursion, read from bit fields of the centred
+// class u (lq bits, sign-extended).  up_at(S) is the round-half-even carry of
+// the residual at weight 2^S -- 1 iff residual_at(r, S) is the negative
+// representative: at S = lq the sign of the class; S <= 0: 0; else bit S-1
+// set and (some lower bit set, or bit S set: the tie goes to even).  With
+// S = lq - j*lb and S1 = max(S - lb, 0), digit j of the recursion is
+//   d_j = bits [S1, S) of u + up_at(S1) - up_at(S) * 2^(S - S1)
+// (the residuals before and after level j differ by d_j * 2^S1).  tz is the
+// number of trailing zero bits of u (64 for u = 0).
+__device__ __forceinline__ int up_at(uint64_t u, int S, int lq, int tz)
+{
+    if (S >= lq)
+        return (int)(u >> (lq - 1)) & 1;
+    if (S <= 0)
+        return 0;
+    return (int)((u >> (S - 1)) & 1) & ((int)(tz < S - 1) | (int)((u >> S) & 1));
+}
+
+__device__ __forceinline__ int trailing_zeros(uint64_t u) { return u ? __builtin_ctzll(u) : 64; }
+
+__device__ __forceinline__ int digit_at(uint64_t u, int j, int lb, int lq, int tz)
+{
+    const int S = lq - j * lb;
+    if (S <= 0)
+        return 0;
+    const int S1 = S > lb ? S - lb : 0;
+    const int field = (int)((u >> S1) & ((1ULL << (S - S1)) - 1));
+    return field + up_at(u, S1, lq, tz) - (up_at(u, S, lq, tz) << (S - S1));
+}
+
 // Torus class of a rounded FP64 transform output (jax_inversefourier's
 // round, then centred_mod q).
 __device__ __forceinline__ int64_t f64_to_torus(double v, double q, double inv_q, int lq)

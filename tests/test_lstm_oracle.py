@@ -77,6 +77,58 @@ def test_digit_recursion_restarts_from_closed_form_residual(log_q, log_beta, l):
                 assert np.array_equal(d, D[j]), (k, j)
 
 
+def _digit_from_bits(x, j, log_beta, log_q):
+    """The closed form csrc/torus_device.h's digit_at computes for level j of
+    the float-rule recursion: with S = log_q - j*log_beta and S1 = max(S -
+    log_beta, 0), d_j = bits [S1, S) of x + up(S1) - up(S) * 2^(S - S1), where
+    up(S) is the round-half-even carry of the residual at weight 2^S: the sign
+    bit at S = log_q, 0 at S <= 0, else bit S-1 and (a lower bit set, or bit S)."""
+    u = x.astype(np.uint64)
+    tz_low = lambda S: (u & np.uint64((1 << (S - 1)) - 1)) != 0
+
+    def up(S):
+        if S >= log_q:
+            return ((u >> np.uint64(log_q - 1)) & np.uint64(1)).astype(np.int64)
+        if S <= 0:
+            return np.zeros(u.shape, np.int64)
+        b = (u >> np.uint64(S - 1)) & np.uint64(1)
+        c = (u >> np.uint64(S)) & np.uint64(1)
+        return (b & (tz_low(S).astype(np.uint64) | c)).astype(np.int64)
+
+    S = log_q - j * log_beta
+    if S <= 0:
+        return np.zeros(u.shape, np.int64)
+    S1 = max(S - log_beta, 0)
+    field = ((u >> np.uint64(S1)) & np.uint64((1 << (S - S1)) - 1)).astype(np.int64)
+    return field + up(S1) - (up(S) << (S - S1))
+
+
+@pytest.mark.parametrize("log_q,log_beta,l", [(64, 2, 20), (64, 13, 2), (64, 8, 8), (32, 7, 3),
+                                              (48, 4, 12), (20, 3, 7), (24, 5, 6), (64, 2, 32),
+                                              (32, 4, 8), (16, 1, 16), (12, 5, 4)])
+def test_digits_from_bit_fields_equal_recursion(log_q, log_beta, l):
+    """The trace-packing and key-switch kernels read each digit of the float
+    rule straight from bit fields of the centred class (no recursion): equal
+    to the oracle's digits at every level, with round-half ties planted at
+    every level and the all-zero / sign-boundary classes included.  A carry
+    that ignored the tie bit, the sticky lower bits, or the sign at the top
+    level fails here."""
+    rng = np.random.default_rng(log_q * 1000 + log_beta * 10 + l)
+    x = L.uniform(rng, 6000, log_q).astype(np.uint64)
+    for j in range(l):
+        s = log_q - (j + 1) * log_beta
+        if s >= 1:
+            sl = slice(j * 150, j * 150 + 300)
+            x[sl] = (x[sl] & ~np.uint64((1 << s) - 1)) | np.uint64(1 << (s - 1))
+            sl2 = slice(j * 150 + 3000, j * 150 + 3050)    # ties with the tie bit's neighbour set
+            x[sl2] = (x[sl2] & ~np.uint64((1 << s) - 1)) | np.uint64(1 << (s - 1)) | np.uint64(1 << s)
+    x = L.tmod(x.view(np.int64), log_q)
+    x[:4] = [0, -(1 << (log_q - 1)) if log_q < 64 else np.iinfo(np.int64).min, -1, 1]
+    D = L.digits(x, log_beta, l, log_q)
+    for j in range(l):
+        assert np.array_equal(_digit_from_bits(x, j, log_beta, log_q), D[j]), j
+
+
 def test_tmod_and_rhe():
     assert L.tmod(np.array([2 ** 31, -2 ** 31 - 1, 5]), 32).tolist() == [-2 ** 31, 2 ** 31 - 1, 5]
     r = np.array([5, 6, 7, -5, -6, -7, 12, -12], np.int64)       # /4: 1.25 1.5 1.75 ...
