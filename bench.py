@@ -35,7 +35,7 @@ INT8_PEAK_TOPS = 5000.0          # dense int8 MFMA
 # scales read off a 1 GiB torch.neg calibration kernel in the same runs
 # (batched: mean of both launches, plain: of the two after the first).
 # None for other batch sizes.
-HBM_TRAFFIC_BYTES = {"batched": 110.73e9 + 1.040e9, "plain": 202.2e9 + 0.604e9}
+HBM_TRAFFIC_BYTES = {"batched": 110.73e9 + 1.040e9, "plain": 202.07e9 + 0.537e9}
 HBM_TRAFFIC_SOURCE = ("profiles/br_kernel_2048_3_pbs16384.md (rocprofv3 --pmc FETCH_SIZE x2 + "
                       "WRITE_SIZE x1, separate runs of bench.py at this workload)")
 
