@@ -181,8 +181,10 @@ __global__ __launch_bounds__(LIN_THREADS) void linear_extract_kernel(
 // 2 * N/2), clamped to [0, slot_count) (batches that mix clients).
 // RG = 2 (N in 512..2048): two groups of N/8 threads per LWE, each keeping one
 // row of the RLWE and transforming every other pair of gadget levels, so a
-// step runs on twice the threads (measured 9-12 % faster than one group from
-// 128 to 4096 LWEs at N = 2048, l = 20).  RG = 1: one group (other N).
+// step runs on twice the threads; the launch uses it below 512 LWEs, where
+// the LWEs alone leave CUs idle.  RG = 1: one group per LWE, two workgroups
+// per CU -- from 512 LWEs at N = 512..2048 (4096 LWEs: 5.4 against 6.8 ms in
+// the two-group form), and at every other N.  Both give bit-identical results.
 template <int N, int RG>
 __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void trace_pack_kernel(
     const int64_t *__restrict__ lwe_a, const int64_t *__restrict__ lwe_b,
@@ -289,11 +291,11 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
         // transforms the level pairs h, h + RG, ..., each pair's digits read
         // straight from bit fields of the class (digit_at)
         const cplx *key = gkey + (size_t)st * l * 2 * M;
-        // digits of level pair P, their transforms, and the products with the
+        // digits of level pair `pair`, their transforms, and the products with the
         // key rows of those levels added into A (in level order)
-        auto pair_mac = [&](int P, cplx (&A)[2][2][2]) {
+        auto pair_mac = [&](int pair, cplx (&A)[2][2][2]) {
             cplx v[2][4];
-            const int lev = 2 * P;
+            const int lev = 2 * pair;
             double dg[2][8];
 #pragma unroll
             for (int i = 0; i < 8; i++)
