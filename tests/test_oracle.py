@@ -219,3 +219,23 @@ def test_get_plain_LUT_closed_form():
             full = np.zeros(N)
             full[np.arange(sub) * n_lut + idx] = vals
             assert coef0 == full[psi + N // 2]
+
+
+@pytest.mark.parametrize("kind,n_lut,msg_bits,N,n", [("identity", 1, 3, 256, 24), ("many", 4, 4, 512, 36),
+                                                   ("many", 2, 3, 128, 12)])
+def test_noise_free_pbs_is_the_lut_at_the_switched_phase(kind, n_lut, msg_bits, N, n):
+    """The exact (int128) oracle's PBS decrypts to the test polynomial read at
+    the modulus-switched phase, sum_g idx[g, p_g] - b_ms (tests/tfhe_cases.py):
+    the expectation the GPU tests use at full size.  Inputs with large LWE
+    noise make many outputs miss LUT(m), so the check is not the trivial one."""
+    from oracle import exact as X
+    from tests.tfhe_cases import decrypt_outputs, expected_mod, lut_at_switched_phase, make_case
+    case = make_case(N=N, n=n, L=3, log_beta=7, B=40, n_lut=n_lut, msg_bits=msg_bits, seed=77,
+                     sigma_bsk=2.0 ** 4, sigma_lwe=2.0 ** 27, kind=kind)
+    idx = O.rotation_indices(case.a, 32, N, 3, n_lut)
+    br = X.blind_rotate(case.lut, case.bsk_coef_k, idx, 7, 32, 3)
+    em, eb = O.cuboot_merge_tail(br, case.b, 32, n_lut)
+    dec, _ = decrypt_outputs(case, em, eb)
+    want = lut_at_switched_phase(case)
+    np.testing.assert_array_equal(dec, want)
+    assert (want != expected_mod(case)).any()        # the noisy inputs do miss LUT(m)

@@ -101,3 +101,33 @@ def decrypt_outputs(case: PBSCase, masks: np.ndarray, bodies: np.ndarray):
 
 def expected_mod(case: PBSCase, B: int | None = None) -> np.ndarray:
     return np.mod(case.expected()[:B], 1 << (case.msg_bits + 1))
+
+
+def switched_rotation(case: PBSCase, B: int | None = None) -> np.ndarray:
+    """(B,) exponent e of the noise-free programmable bootstrap: the blind
+    rotation multiplies the test polynomial by X^idx[g, p_g] for every group g
+    of c secret bits (p_g = the group's bit pattern, most significant first,
+    as all_binary_vectors orders them), then the extraction rotates by
+    -b_ms: e = sum_g idx[g, p_g] - b_ms mod 2N."""
+    Bn = len(case.b) if B is None else B
+    c = case.collapse
+    n_iter = case.n // c
+    idx = O.rotation_indices(case.a[:Bn], case.log_q, case.N, c, case.n_lut)
+    s = np.asarray(case.s[: n_iter * c]).reshape(n_iter, c)
+    p = (s * (1 << (c - 1 - np.arange(c)))).sum(axis=1)
+    e = idx[:, np.arange(n_iter), p].astype(np.int64).sum(axis=1)
+    bms = O.body_rotation(case.b[:Bn], case.log_q, case.N, case.n_lut).reshape(-1)
+    return np.mod(e - bms, 2 * case.N)
+
+
+def lut_at_switched_phase(case: PBSCase, B: int | None = None) -> np.ndarray:
+    """(B, n_lut) messages the bootstrap outputs carry without noise:
+    coefficients 0..n_lut-1 of X^e * T (T = the test polynomial's body row),
+    over Delta_out, mod 2^(msg_bits+1).  The output noise (the key's) is far
+    below Delta_out/2 in these cases, so decryptions must equal these exactly;
+    whether they also equal LUT(m) depends on the modulus switch alone."""
+    e = switched_rotation(case, B)
+    T = np.asarray(case.lut[1], np.int64)
+    vals = np.stack([O.coef_rotation(T, int(x))[: case.n_lut] for x in e])
+    mod = 1 << (case.msg_bits + 1)
+    return np.mod(np.rint(vals / case.delta_out).astype(np.int64), mod)
