@@ -26,10 +26,14 @@
 #include <string.h>
 
 typedef __int128 i128;
+/* sums wrap mod 2^128 (well defined for an unsigned type); every result is
+   read mod 2^log_q <= 2^64, which that wrap preserves, and the blind
+   rotation's sums (small digits times key entries) never wrap at all */
+typedef unsigned __int128 u128;
 
-static int64_t centred(i128 v, int log_q)
+static int64_t centred(u128 v, int log_q)
 {
-    uint64_t r = (uint64_t)v; /* v mod 2^64, two's complement */
+    uint64_t r = (uint64_t)v; /* v mod 2^64 */
     if (log_q >= 64)
         return (int64_t)r;
     uint64_t q = 1ULL << log_q;
@@ -60,7 +64,7 @@ static void decompose(const int64_t *ct, int64_t *dec, int N, int L, int log_bet
 }
 
 /* out += X^e * p  (negacyclic, e in [0, 2N)) */
-static void add_rotated(i128 *out, const int64_t *p, int e, int N)
+static void add_rotated(u128 *out, const int64_t *p, int e, int N)
 {
     for (int k = 0; k < N; k++) {
         int t = k + e;
@@ -69,28 +73,28 @@ static void add_rotated(i128 *out, const int64_t *p, int e, int N)
             t -= N;
             neg ^= 1;
         }
-        out[t] += neg ? -(i128)p[k] : (i128)p[k];
+        out[t] += neg ? (u128)0 - (u128)(i128)p[k] : (u128)(i128)p[k];
     }
 }
 
 /* acc += a * d  (negacyclic), a centred mod q, d small */
-static void negacyclic_mac(i128 *acc, const int64_t *a, const int64_t *d, int N)
+static void negacyclic_mac(u128 *acc, const int64_t *a, const int64_t *d, int N)
 {
     for (int i = 0; i < N; i++) {
         i128 di = d[i];
         if (di == 0)
             continue;
         for (int j = 0; j < N - i; j++)
-            acc[i + j] += di * a[j];
+            acc[i + j] += (u128)(di * a[j]);   /* |product| < 2^126: exact */
         for (int j = N - i; j < N; j++)
-            acc[i + j - N] -= di * a[j];
+            acc[i + j - N] -= (u128)(di * a[j]);
     }
 }
 
 /* Exact negacyclic product mod 2^log_q (centred). */
 void tfhe_exact_negacyclic_mul(const int64_t *a, const int64_t *b, int64_t *out, int N, int log_q)
 {
-    i128 *acc = (i128 *)calloc((size_t)N, sizeof(i128));
+    u128 *acc = (u128 *)calloc((size_t)N, sizeof(u128));
     negacyclic_mac(acc, a, b, N);
     for (int k = 0; k < N; k++)
         out[k] = centred(acc[k], log_q);
@@ -112,17 +116,17 @@ void tfhe_exact_blind_rotate(const int64_t *ct_in, long stride_in, const int64_t
         int64_t *ct = (int64_t *)malloc(sizeof(int64_t) * 2 * N);
         int64_t *dec = (int64_t *)malloc(sizeof(int64_t) * (size_t)L * 2 * N);
         int64_t *P = (int64_t *)malloc(sizeof(int64_t) * N);
-        i128 *rot = (i128 *)malloc(sizeof(i128) * N);
-        i128 *acc = (i128 *)malloc(sizeof(i128) * 2 * N);
+        u128 *rot = (u128 *)malloc(sizeof(u128) * N);
+        u128 *acc = (u128 *)malloc(sizeof(u128) * 2 * N);
         memcpy(ct, ct_in + (size_t)b * stride_in, sizeof(int64_t) * 2 * N);
         const int BC = 2 * L;
         for (int g = 0; g < n_iter; g++) {
             const int32_t *id = idx + ((size_t)b * n_iter + g) * m;
             decompose(ct, dec, N, L, log_beta, log_q);
-            memset(acc, 0, sizeof(i128) * 2 * N);
+            memset(acc, 0, sizeof(u128) * 2 * N);
             for (int a = 0; a < 2; a++) {
                 for (int bc = 0; bc < BC; bc++) {
-                    memset(rot, 0, sizeof(i128) * N);
+                    memset(rot, 0, sizeof(u128) * N);
                     for (int p = 0; p < m; p++) {
                         const int64_t *key =
                             bsk + ((((size_t)g * 2 + a) * BC + bc) * m + p) * (size_t)N;

@@ -239,3 +239,20 @@ def test_noise_free_pbs_is_the_lut_at_the_switched_phase(kind, n_lut, msg_bits, 
     want = lut_at_switched_phase(case)
     np.testing.assert_array_equal(dec, want)
     assert (want != expected_mod(case)).any()        # the noisy inputs do miss LUT(m)
+
+
+def test_c_oracle_clean_under_address_and_ub_sanitizers():
+    """The exact C oracle built with -fsanitize=address,undefined (host code,
+    gcc) runs its own checks (products against a schoolbook, broadcast vs
+    per-ciphertext blind rotation, zero key) without a sanitizer report."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None or shutil.which("make") is None:
+        pytest.skip("gcc/make not available")
+    root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
+    r = subprocess.run(["make", "-s", "-f", "oracle/sanitize.mk"], cwd=root, capture_output=True,
+                       text=True, timeout=300)
+    out = r.stdout + r.stderr
+    assert r.returncode == 0, out[-3000:]
+    assert "sanitize check ok" in out
+    assert "runtime error" not in out and "ERROR: AddressSanitizer" not in out
