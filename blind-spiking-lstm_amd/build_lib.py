@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbslstm_tfhe.so")
-SOURCES = ["capi.hip", "blind_rotate.hip", "pbs_ops.hip", "lstm_ops.hip", "ks_mfma.hip", "split_br.hip"]
+SOURCES = ["capi.hip", "blind_rotate.hip", "br_pair.hip", "pbs_ops.hip", "lstm_ops.hip", "ks_mfma.hip", "split_br.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",

@@ -33,6 +33,11 @@ SIGNATURES = {
                                       _I64, _I64, _I64, _I64], ctypes.c_int),
     "bslstm_mul_bootstrap_rows": ([_P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _I64, _P, _I64,
                                    _I64, _I64, _I64, _I64], ctypes.c_int),
+    "bslstm_mul_bootstrap_pair_workspace_bytes": ([_I64, _I64, _I64], ctypes.c_int64),
+    "bslstm_mul_bootstrap_pair": ([_P, _P, _I64, _P, _P, _P, _I64, _I64, _P, _I64, _P, _I64,
+                                   _I64, _I64, _I64, _I64, _P, _I64, _I64], ctypes.c_int),
+    "bslstm_mul_bootstrap_pair_slots": ([_P, _P, _I64, _P, _P, _P, _I64, _P, _I64, _P, _I64,
+                                         _I64, _I64, _I64, _I64, _P, _I64, _I64], ctypes.c_int),
     "bslstm_rotation_indices": ([_P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P], ctypes.c_int),
     "bslstm_rotate_extract": ([_P, _P, _P, _I64, _I64, _I64, _I64, _P, _P], ctypes.c_int),
     "bslstm_key_switch_lwe": ([_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P, _P],

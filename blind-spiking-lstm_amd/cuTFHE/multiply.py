@@ -55,7 +55,54 @@ def prepare_bsk(BSK: torch.Tensor, n: int, collapse: int) -> torch.Tensor:
     return BSK.reshape(shape).permute(3, 0, 2, 1, 4, 5).contiguous()
 
 
-FORMS = ("plain", "batched", "rows")
+FORMS = ("plain", "batched", "rows", "pair")
+
+# polls of a pair-form workgroup waiting on its partner before it gives up
+# (each poll sleeps ~50 ns and reads the flag, ~0.5-2 s in all; a launch whose
+# workgroups are all resident never comes near this)
+PAIR_SPIN_LIMIT = 1 << 20
+_PAIR_LAST: dict = {}      # device index -> workspace of the last pair-form call
+
+
+def pair_applicable(degree: int, l: int, batch: int, device=None) -> bool:
+    """Whether the pair form takes this shape: degree 2048, l <= 3 and every
+    workgroup resident at once (two CUs per ciphertext, batch rounded up to 8)."""
+    if degree != 2048 or not 1 <= l <= 3 or batch < 1:
+        return False
+    cus = torch.cuda.get_device_properties(device).multi_processor_count
+    return (batch + 7) // 8 * 8 <= cus // 2
+
+
+def pair_status(device=None) -> int:
+    """Status word of the last pair-form call on `device` (synchronises):
+    bit 0 = a workgroup's wait on its partner ran out (that call's outputs are
+    wrong), bit 1 = some pair ran on two XCDs; 0 when no call was made."""
+    dev = torch.cuda.current_device() if device is None else torch.device(device).index
+    ws = _PAIR_LAST.get(dev)
+    return 0 if ws is None else int(ws[:4].view(torch.int32).item())
+
+
+def _pair_call(ct, bsk, arr, rot, idx, out, batch, log_beta, log_q, l, degree, slots=None,
+               slot_count=0, spin_limit=None):
+    ws_bytes = int(_core.lib().bslstm_mul_bootstrap_pair_workspace_bytes(batch, l, degree))
+    if ws_bytes < 0:
+        raise _core.TFHEInvalidArgument(
+            f"the pair form runs at degree 2048 and l <= 3 (got N={degree}, l={l})")
+    ws = torch.empty(max(ws_bytes, 4), dtype=torch.int8, device=ct.device)
+    spin = PAIR_SPIN_LIMIT if spin_limit is None else int(spin_limit)
+    if slots is None:
+        rot_rows, rot_cols = (0, 0) if rot is None else (rot.shape[-2], rot.shape[-1])
+        _core.check(_core.lib().bslstm_mul_bootstrap_pair(
+            _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr, _core.ptr(rot),
+            rot_rows, rot_cols, idx.data_ptr(), idx.numel(), out.data_ptr(), batch, log_beta,
+            log_q, l, degree, ws.data_ptr(), ws_bytes, spin))
+    else:
+        _core.check(_core.lib().bslstm_mul_bootstrap_pair_slots(
+            _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr, slots.data_ptr(),
+            slot_count, idx.data_ptr(), idx.numel(), out.data_ptr(), batch, log_beta, log_q, l,
+            degree, ws.data_ptr(), ws_bytes, spin))
+    _PAIR_LAST[ct.device.index] = ws
+
 
 SPLIT_CHUNK = 512      # ciphertexts per chunk of the split form's step scratch
 
@@ -161,7 +208,7 @@ def load_tuning(path: str) -> int:
 def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft: torch.Tensor | None,
                   indices: torch.Tensor, log_beta: int, log_q: int, l: int,
                   form: str = "batched", slots: torch.Tensor | None = None,
-                  candidates: tuple = FORMS) -> torch.Tensor:
+                  candidates: tuple = FORMS, spin_limit: int | None = None) -> torch.Tensor:
     """The raw custom call of cuTFHE/multiply.py:17-51 (handler
     csrc/mul_bootstrap.cuh:192-243): int64 ct (…,2,N) -> int64 (batch,2,N).
 
@@ -169,6 +216,10 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     shape); "batched": two ciphertexts per workgroup sharing every key load;
     "rows": one ciphertext per workgroup with its two accumulator rows on two
     thread groups (the latency form for small batches);
+    "pair": two workgroups (CUs) per ciphertext, one accumulator row each,
+    exchanging the digit spectra once per step (degree 2048, l <= 3, batch up
+    to CUs/2; the lower-latency form for a few dozen ciphertexts; spin_limit
+    bounds each wait on the partner, see pair_status);
     "split": exact integer products as int8 MFMA GEMMs (log_q <= 32; the key
     is prepared once per key tensor and cached), equal to the exact product;
     "auto": the autotuner's choice for this shape (timed on first use; an
@@ -179,10 +230,14 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     (C, n_iter, 2, l, 2, m, N/2) of client keys and ciphertext i uses
     pool[slots[i]]; in the batched form consecutive pairs must share a slot
     (callers whose pairs may not share one tune over candidates=("plain",
-    "rows"))."""
+    "rows", "pair")).  Candidates that do not take the shape ("pair" outside
+    its range) are left out of the tuning."""
     if form == "auto":
         dims = tuple(int(d) for d in bsk_reshaped.shape[-6:])
         batch = indices.numel() // (dims[0] * dims[4]) if dims[0] * dims[4] else 0
+        if "pair" in candidates and not pair_applicable(ct_init.shape[-1], l, batch,
+                                                        ct_init.device):
+            candidates = tuple(c for c in candidates if c != "pair") or ("rows",)
         key = (ct_init.shape[-1], l, dims[0], dims[4], _bucket(batch), slots is not None,
                tuple(candidates))
         if key in _TUNED:
@@ -236,12 +291,20 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
         sl = slots.to(torch.int32).reshape(-1).contiguous()
         if bsk.dim() != 7 or sl.numel() != batch:
             raise _core.TFHEInvalidArgument("slots need a (C, 6-D) key pool and one slot per ciphertext")
+        if form == "pair":
+            _pair_call(ct, bsk, arr, None, idx, out, batch, log_beta, log_q, l, degree, sl,
+                       bsk.shape[0], spin_limit)
+            return out
         _core.check(_core.lib().bslstm_mul_bootstrap_slots(
             _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr, sl.data_ptr(),
             bsk.shape[0], idx.data_ptr(), idx.numel(), out.data_ptr(), batch, log_beta, log_q, l,
             degree, _FORM_CODE[form]))
         return out
     rot = None if all_rot_fft is None else all_rot_fft.contiguous()
+    if form == "pair":
+        _pair_call(ct, bsk, arr, rot, idx, out, batch, log_beta, log_q, l, degree,
+                   spin_limit=spin_limit)
+        return out
     rot_rows, rot_cols = (0, 0) if rot is None else (rot.shape[-2], rot.shape[-1])
     entry = {"plain": _core.lib().bslstm_mul_bootstrap,
              "batched": _core.lib().bslstm_mul_bootstrap_batched,

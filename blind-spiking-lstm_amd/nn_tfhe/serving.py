@@ -166,7 +166,7 @@ class SlotBootstrapper:
         lut = torch.stack([torus._i64(encrypted_LUT[0]), torus._i64(encrypted_LUT[1])], dim=1)
         br = mul_bootstrap(lut, self.pool.pools["bsk"], None, idx, log2_int(p["beta_bs"]),
                            log2_int(p["q"]), p["l_bs"], form="auto", slots=self._per(rows),
-                           candidates=FORMS if per_seq % 2 == 0 else ("plain", "rows"))
+                           candidates=FORMS if per_seq % 2 == 0 else ("plain", "rows", "pair"))
         masks, bodies = torus.rotate_extract(br, b_ks.reshape(rows), log2_int(p["q"]), self.n_lut)
         return masks, bodies[..., None]
 

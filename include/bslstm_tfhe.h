@@ -100,6 +100,47 @@ int bslstm_mul_bootstrap_rows(void *stream,
                               int64_t *output, int64_t batch,
                               int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree);
 
+/*
+ * The "pair" latency form: same contract, operands, errors and bit-identical
+ * outputs as bslstm_mul_bootstrap (csrc/mul_bootstrap.cuh:138-243), over two
+ * workgroups (two CUs) per ciphertext.  Each workgroup keeps one accumulator
+ * row, transforms that row's gadget digits, exchanges the spectra with its
+ * partner through device memory once per CMux step and multiplies half of
+ * the step's key slice: for a few dozen ciphertexts, where one CU per
+ * ciphertext is bound by reading the key slice from L2.
+ *
+ * Degree 2048 and l <= 3 only; at most (CU count / 2) ciphertexts (rounded
+ * up to a multiple of 8), so that every workgroup is resident at once.
+ * workspace: device memory of bslstm_mul_bootstrap_pair_workspace_bytes()
+ * bytes, cleared by the call itself (stream-ordered; one call at a time per
+ * workspace).  Its first 4 bytes are a status word the caller may read after
+ * the call: bit 0 = a wait on the partner workgroup ran out after
+ * spin_limit polls (the outputs are then wrong; the kernel still ends),
+ * bit 1 = some pair ran on two XCDs (slower coherent exchange, same result).
+ * spin_limit in [1, 2^30]; each poll sleeps ~50 ns.
+ */
+int64_t bslstm_mul_bootstrap_pair_workspace_bytes(int64_t batch, int64_t l, int64_t degree);
+
+int bslstm_mul_bootstrap_pair(void *stream,
+                              const int64_t *ct_init, int64_t ct_elems,
+                              const double *bsk, const int64_t bsk_dims[6],
+                              const double *all_rot_fft, int64_t rot_rows, int64_t rot_cols,
+                              const int32_t *indices, int64_t idx_elems,
+                              int64_t *output, int64_t batch,
+                              int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree,
+                              void *workspace, int64_t workspace_bytes, int64_t spin_limit);
+
+/* bslstm_mul_bootstrap_pair with bsk_pool (slot_count, n_iter, 2, l, 2, m,
+ * N/2) and slots int32 (batch), as bslstm_mul_bootstrap_slots. */
+int bslstm_mul_bootstrap_pair_slots(void *stream,
+                                    const int64_t *ct_init, int64_t ct_elems,
+                                    const double *bsk_pool, const int64_t bsk_dims[6],
+                                    const int32_t *slots, int64_t slot_count,
+                                    const int32_t *indices, int64_t idx_elems,
+                                    int64_t *output, int64_t batch,
+                                    int64_t log2_beta, int64_t log2_q, int64_t l, int64_t degree,
+                                    void *workspace, int64_t workspace_bytes, int64_t spin_limit);
+
 /* ------------------------------------------------------------------------
  * Blind rotation in exact integer arithmetic on the matrix cores (the
  * "split" form): the same computation as bslstm_mul_bootstrap (the handler

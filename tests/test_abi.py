@@ -144,3 +144,36 @@ def test_split_form_entry_points_reject_on_host():
     assert call(idx_elems=5) == bad and "indices" in lib.bslstm_last_error().decode()
     assert call(ws_bytes=ws - 1) == bad and "workspace" in lib.bslstm_last_error().decode()
     assert call(batch=0, idx_elems=0) == _core.BSLSTM_OK
+
+
+def test_pair_form_entry_points_reject_on_host():
+    """Validation of the pair-form entry points runs before any GPU work."""
+    lib = _core.lib()
+    bad = _core.BSLSTM_INVALID_ARGUMENT
+    dims = (_core._I64 * 6)(250, 2, 3, 2, 8, 1024)
+    B = 4
+    # header (status word, flags, hello) padded to 256 bytes, then 8 pairs x
+    # 2 step parities x 2 members x l x N/2 complex
+    ws = lib.bslstm_mul_bootstrap_pair_workspace_bytes(B, 3, 2048)
+    assert ws == 256 + 8 * 2 * 2 * 3 * 1024 * 16
+    assert lib.bslstm_mul_bootstrap_pair_workspace_bytes(B, 4, 2048) == -1
+    assert lib.bslstm_mul_bootstrap_pair_workspace_bytes(B, 3, 1024) == -1
+
+    def call(**kw):
+        a = dict(ct_elems=4096, idx_elems=B * 250 * 8, batch=B, lb=7, lq=32, l=3, degree=2048,
+                 ws_bytes=ws, spin=1000, dims=dims)
+        a.update(kw)
+        return lib.bslstm_mul_bootstrap_pair(None, 8, a["ct_elems"], 8, a["dims"], None, 0, 0, 8,
+                                             a["idx_elems"], 8, a["batch"], a["lb"], a["lq"],
+                                             a["l"], a["degree"], 8, a["ws_bytes"], a["spin"])
+    assert call(degree=1024, ct_elems=2048, dims=(_core._I64 * 6)(250, 2, 3, 2, 8, 512)) == bad
+    assert "degree 2048" in lib.bslstm_last_error().decode()
+    assert call(l=4, dims=(_core._I64 * 6)(250, 2, 4, 2, 8, 1024)) == bad
+    assert "l <= 3" in lib.bslstm_last_error().decode()
+    assert call(spin=0) == bad and "spin_limit" in lib.bslstm_last_error().decode()
+    assert call(spin=(1 << 30) + 1) == bad
+    assert call(ws_bytes=ws - 1) == bad and "workspace" in lib.bslstm_last_error().decode()
+    assert call(idx_elems=5) == bad and "indices" in lib.bslstm_last_error().decode()
+    assert call(batch=0, idx_elems=0) == _core.BSLSTM_OK
+    assert lib.bslstm_mul_bootstrap_pair_slots(None, 8, 4096, 8, dims, None, 2, 8, B * 250 * 8, 8,
+                                               B, 7, 32, 3, 2048, 8, ws, 1000) == bad
