@@ -235,7 +235,7 @@ def run_form_sweep():
     inputs (the time does not depend on their values); 3-launch minimum."""
     import torch
     sys.path.insert(0, PKG)
-    from cuTFHE.multiply import mul_bootstrap
+    from cuTFHE.multiply import mul_bootstrap, pair_applicable
     dev = torch.device("cuda:0")
     N = 2048
     out = {}
@@ -249,7 +249,9 @@ def run_form_sweep():
             ct = torch.randint(-2 ** 31, 2 ** 31, (B, 2, N), dtype=torch.int64, generator=g).to(dev)
             idx = torch.randint(0, 2 * N, (B, n_iter, 8), dtype=torch.int32, generator=g).to(dev)
             ms = {}
-            for form in ("plain", "batched", "rows"):
+            forms = ("plain", "batched", "rows") + \
+                (("pair",) if pair_applicable(N, L, B, dev) else ())
+            for form in forms:
                 mul_bootstrap(ct, bsk, None, idx, lb, lq, L, form=form)
                 ts = []
                 for _ in range(3):

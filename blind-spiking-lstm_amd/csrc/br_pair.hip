@@ -8,10 +8,16 @@
 // of a pair keeps accumulator row a and, per step:
 //   1. transforms the gadget digits of its row (its two thread groups take
 //      alternate levels), into LDS;
-//   2. publishes those spectra to its partner through L2 and reads the
-//      partner's (the other row's digit spectra) -- one exchange per step;
-//   3. multiplies the key rows of output row a only: half the key slice;
-//   4. inverts its row (both thread groups, each on its own copy, so both
+//   2. publishes those spectra to its partner through L2: thread u writes
+//      the bins it multiplies (u and M-1-u), which are the bins the
+//      partner's thread u needs;
+//   3. streams the key rows of output row a (half the key slice) against the
+//      rotation roots -- work that does not need any digit spectrum, so the
+//      partner's spectra travel meanwhile;
+//   4. waits for the partner's flag, loads the partner row's spectra at its
+//      bins into registers and combines: for each level, sum over the two
+//      rows of (key patterns x spectrum), in the one-workgroup forms' order;
+//   5. inverts its row (both thread groups, each on its own copy, so both
 //      keep the row in registers for the next step's digits).
 // The arithmetic per output coefficient is the one-workgroup forms' (same
 // digits, transforms, pattern and level order), so the results are identical.
@@ -42,7 +48,7 @@ struct PairWs {
     uint32_t *status;    // bit 0: a wait ran out; bit 1: a pair straddled two XCDs
     uint32_t *flags;     // [pair][member]: steps published
     uint32_t *hello;     // [pair][member]: XCC_ID + 1
-    cplx *xbuf;          // [pair][parity][member][L][M]
+    cplx *xbuf;          // [pair][parity][member][L][M/2 threads][2 bins]
 };
 
 static int64_t pair_count(int64_t batch) { return (batch + 7) / 8 * 8; }
@@ -174,7 +180,12 @@ __global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
     const int lpw = (lines + P - 1) / P;
     const int line0 = ((b >> 4) % P) * lpw;
     const int line_end = min(lines, line0 + lpw);
-    constexpr int PF = 4;                          // warm-up loads per thread per step
+    // warm-up loads per thread per step: the whole half slice (4096 lines at
+    // l = 2, 6144 at l = 3, m = 8) from two workgroups per row and XCD on;
+    // a workgroup warming alone takes four lines per thread (half the slice:
+    // measured faster than all of it)
+    constexpr int PF = L <= 2 ? 8 : 6;
+    const int npf = P == 1 ? 4 : PF;
     uint32_t warm = 0, pf[PF] = {};
 
     cplx *const xb_mine0 = ws.xbuf + (((size_t)pair * 2 + 0) * 2 + a) * L * M;
@@ -202,19 +213,29 @@ __global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
             forward_wave<N, FULL, 1>(v1, lev < L ? own + lev * M : oth, ztab, t);
         }
 
-        // ---- 2. exchange the spectra with the partner (step parity buffers)
-        cplx *xb_mine = xb_mine0 + (g & 1) * parity_stride;
-        const cplx *xb_other = xb_other0 + (g & 1) * parity_stride;
-        for (int i = threadIdx.x; i < L * M; i += blockDim.x) {
-            const cplx v = own[i];
+        // ---- 2. publish this row's spectra at the bins thread u multiplies
+        // (bin pair (u, M-1-u)), double-buffered by step parity
+        const int r = u;
+        cplx *xb_mine = xb_mine0 + (g & 1) * parity_stride + 2 * r;
+        const cplx *xb_other = xb_other0 + (g & 1) * parity_stride + 2 * r;
+        cplx dm[L][2];
+#pragma unroll
+        for (int lev = 0; lev < L; lev++) {
+            dm[lev][0] = own[lev * M + phys<N>(r)];
+            dm[lev][1] = own[lev * M + phys<N>(M - 1 - r)];
             if (local) {
-                xb_mine[i] = v;
+                xb_mine[lev * M] = dm[lev][0];
+                xb_mine[lev * M + 1] = dm[lev][1];
             } else {
-                unsigned long long *d = reinterpret_cast<unsigned long long *>(xb_mine + i);
-                __hip_atomic_store(d, (unsigned long long)__double_as_longlong(v.re), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_SYSTEM);
-                __hip_atomic_store(d + 1, (unsigned long long)__double_as_longlong(v.im),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                unsigned long long *d = reinterpret_cast<unsigned long long *>(xb_mine + lev * M);
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    __hip_atomic_store(d + 2 * k, (unsigned long long)__double_as_longlong(dm[lev][k].re),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __hip_atomic_store(d + 2 * k + 1,
+                                       (unsigned long long)__double_as_longlong(dm[lev][k].im),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
             }
         }
         // every wave's stores must have reached L2 (same XCD) or memory
@@ -222,71 +243,78 @@ __global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
         // waits for LDS, so wait for all of this wave's memory operations
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (threadIdx.x == 0 && !broken) {
-            bool ok;
-            if (local) {
-                __hip_atomic_store(&ws.flags[pair * 2 + a], (uint32_t)(g + 1), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                ok = wait_at_least<__HIP_MEMORY_SCOPE_AGENT>(&ws.flags[pair * 2 + 1 - a],
-                                                             (uint32_t)(g + 1), spin_limit);
-            } else {
-                __hip_atomic_store(&ws.flags[pair * 2 + a], (uint32_t)(g + 1), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_SYSTEM);
-                ok = wait_at_least<__HIP_MEMORY_SCOPE_SYSTEM>(&ws.flags[pair * 2 + 1 - a],
-                                                              (uint32_t)(g + 1), spin_limit);
-            }
-            broken = !ok;
-            if (!ok)
-                atomicOr(ws.status, 1u);
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < L * M; i += blockDim.x) {
-            const unsigned long long *s = reinterpret_cast<const unsigned long long *>(xb_other + i);
-            unsigned long long re, im;
-            if (local) {   // the L2 is shared: bypass only the L1
-                re = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                im = __hip_atomic_load(s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                re = __hip_atomic_load(s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                im = __hip_atomic_load(s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            }
-            oth[i] = {__longlong_as_double((long long)re), __longlong_as_double((long long)im)};
-        }
-        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&ws.flags[pair * 2 + a], (uint32_t)(g + 1), __ATOMIC_RELAXED,
+                               local ? __HIP_MEMORY_SCOPE_AGENT : __HIP_MEMORY_SCOPE_SYSTEM);
 
-        // ---- 3. external product for output row a (mul_bootstrap.cuh:42-67)
-        const int r = u;                            // bin pair (r, M-1-r)
-        cplx prod[2] = {{0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll 1
+        // ---- 3. key patterns of output row a (mul_bootstrap.cuh:42-67): the
+        // key rows times the rotation roots, per level and input row
+        cplx W[L][1][2][1][2];
+#pragma unroll
         for (int lev = 0; lev < L; lev++) {
-            cplx W[1][2][1][2];
 #pragma unroll
             for (int b2 = 0; b2 < 2; b2++)
-                W[0][b2][0][0] = W[0][b2][0][1] = {0.0, 0.0};
+                W[lev][0][b2][0][0] = W[lev][0][b2][0][1] = {0.0, 0.0};
             cplx bkA[1][2][2], bkB[1][2][2];
             load_pair<L, 1>(bkA, bsk_g, 2 * r, a, lev, 0, m, M);
 #pragma unroll 1
             for (int p = 0; p < m; p += 2) {
                 load_pair<L, 1>(bkB, bsk_g, 2 * r, a, lev, p + 1 < m ? p + 1 : p, m, M);
-                mac_pattern<N, 1, 1, FULL>(W, bkA, idp, p, r, ztab);
+                mac_pattern<N, 1, 1, FULL>(W[lev], bkA, idp, p, r, ztab);
                 if (p + 1 < m) {
                     load_pair<L, 1>(bkA, bsk_g, 2 * r, a, lev, p + 2 < m ? p + 2 : p + 1, m, M);
-                    mac_pattern<N, 1, 1, FULL>(W, bkB, idp, p + 1, r, ztab);
+                    mac_pattern<N, 1, 1, FULL>(W[lev], bkB, idp, p + 1, r, ztab);
                 }
             }
+        }
+
+        // ---- 4. the partner row's spectra, then the sum over levels and rows
+        if (threadIdx.x == 0 && !broken) {
+            const bool ok =
+                local ? wait_at_least<__HIP_MEMORY_SCOPE_AGENT>(&ws.flags[pair * 2 + 1 - a],
+                                                                (uint32_t)(g + 1), spin_limit)
+                      : wait_at_least<__HIP_MEMORY_SCOPE_SYSTEM>(&ws.flags[pair * 2 + 1 - a],
+                                                                 (uint32_t)(g + 1), spin_limit);
+            broken = !ok;
+            if (!ok)
+                atomicOr(ws.status, 1u);
+        }
+        __syncthreads();
+        cplx dp[L][2];
+#pragma unroll
+        for (int lev = 0; lev < L; lev++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const unsigned long long *sp =
+                    reinterpret_cast<const unsigned long long *>(xb_other + lev * M + k);
+                unsigned long long re, im;
+                if (local) {   // the L2 is shared: bypass only the L1
+                    re = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    im = __hip_atomic_load(sp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    re = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    im = __hip_atomic_load(sp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+                dp[lev][k] = {__longlong_as_double((long long)re), __longlong_as_double((long long)im)};
+            }
+        cplx prod[2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+        for (int lev = 0; lev < L; lev++) {
             cplx acc0 = {0.0, 0.0}, acc1 = {0.0, 0.0};
 #pragma unroll
             for (int b2 = 0; b2 < 2; b2++) {
-                const cplx *spec = (b2 == a ? own : oth) + lev * M;
-                const cplx d0 = spec[phys<N>(r)];
-                const cplx d1 = spec[phys<N>(M - 1 - r)];
-                acc0 = cfma(W[0][b2][0][0], d0, acc0);
-                acc1 = cfma(W[0][b2][0][1], d1, acc1);
+                const cplx d0 = b2 == a ? dm[lev][0] : dp[lev][0];
+                const cplx d1 = b2 == a ? dm[lev][1] : dp[lev][1];
+                acc0 = cfma(W[lev][0][b2][0][0], d0, acc0);
+                acc1 = cfma(W[lev][0][b2][0][1], d1, acc1);
             }
             prod[0] = cadd(prod[0], acc0);
             prod[1] = cadd(prod[1], acc1);
         }
 
+        // L2 warm-up of the next step's half slice.  Issued here because the
+        // memory counter retires in order: until the next step's publish only
+        // LDS and VALU work follows, so no wait on these loads stalls a step.
 #pragma unroll
         for (int i = 0; i < PF; i++) {
             warm ^= pf[i];
@@ -298,14 +326,14 @@ __global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
 #pragma unroll
             for (int i = 0; i < PF; i++) {
                 const int j = line0 + (int)threadIdx.x + i * (int)blockDim.x;
-                if (j < line_end)
+                if (i < npf && j < line_end)
                     pf[i] = nxt[(size_t)j * 32];
             }
         }
 
-        // ---- 4. inverse transform of row a, untwist, centred mod q, round
-        // (mul_bootstrap.cuh:70-93); one copy of the products per group
-        __syncthreads();
+        // ---- 5. inverse transform of row a, untwist, centred mod q, round
+        // (mul_bootstrap.cuh:70-93); one copy of the products per group.  No
+        // thread reads own/oth between the barrier above and these writes.
         {
             cplx *c0 = own, *c1 = oth;
             c0[phys<N>(r)] = prod[0];
@@ -346,10 +374,9 @@ __global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
                 x[rr + 4] = (acc_t)(int64_t)hi;
             }
         }
-        // the next step's digits overwrite own/oth only after every thread
-        // has read its inverse input (forward_wave starts with a barrier-free
-        // register pass, then writes LDS after its first barrier)
-        __syncthreads();
+        // no barrier here: the next step's forward transforms write own/oth
+        // only after their own first barrier, which every thread reaches
+        // after reading its inverse input
     }
 
 #pragma unroll
