@@ -96,9 +96,9 @@ def test_mul_bootstrap_other_moduli(cuda_device, mult, log_q, L, lb, tol, form):
 @pytest.mark.parametrize("N,L,c", [(2048, 3, 3), (1024, 2, 2), (512, 4, 1), (256, 3, 4),
                                    (512, 1, 3)])
 def test_all_forms_bit_identical(cuda_device, mult, N, L, c):
-    """plain, batched and rows give identical outputs.  Odd batch (a tail
-    workgroup with one ciphertext) and per-ciphertext test polynomials;
-    collapse 1..4 (m = 2..16 patterns)."""
+    """plain, batched and rows (and pair, at N = 2048 and l <= 3) give
+    identical outputs.  Odd batch (a tail workgroup with one ciphertext) and
+    per-ciphertext test polynomials; collapse 1..4 (m = 2..16 patterns)."""
     case = make_case(N=N, n=12 * c, L=L, log_beta=7, collapse=c, B=9, seed=3 * N + c)
     rng = np.random.default_rng(c)
     cts = rng.integers(-2 ** 31, 2 ** 31, size=(9, 2, N))
@@ -109,6 +109,10 @@ def test_all_forms_bit_identical(cuda_device, mult, N, L, c):
     r_rows = host(mult.mul_bootstrap(dev(cts), bsk, None, idx, 7, 32, L, form="rows"))
     np.testing.assert_array_equal(r_plain, r_batch)
     np.testing.assert_array_equal(r_plain, r_rows)
+    if mult.pair_applicable(N, L, 9, 0):
+        r_pair = host(mult.mul_bootstrap(dev(cts), bsk, None, idx, 7, 32, L, form="pair"))
+        assert mult.pair_status() & 1 == 0
+        np.testing.assert_array_equal(r_plain, r_pair)
     exact = X.blind_rotate(cts, case.bsk_coef_k, host(idx), 7, 32, L)
     assert np.abs(O.centered_mod(r_batch - exact, 32)).max() <= BR_COEF_TOL
 
