@@ -275,11 +275,11 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
         // r: digits source A(X^k); the B row takes B(X^k) now (B <- B + B(X^k)
         // + B_ks, the sum wraps mod 2^64)
         int64_t r[8];
-        int tz[8];
+        uint64_t cv[8];     // every level's carry bit of the class (carries_of)
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             r[i] = tmod((uint64_t)xb[t + i * T], lq);
-            tz[i] = trailing_zeros((uint64_t)r[i]);
+            cv[i] = carries_of((uint64_t)r[i], lq);
         }
 #pragma unroll
         for (int c = 0; c < NR; c++)
@@ -289,7 +289,8 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
                     X[c][i] += (uint64_t)xb[N + t + i * T];
         // ---- gadget product with the Galois key of this step; group h
         // transforms the level pairs h, h + RG, ..., each pair's digits read
-        // straight from bit fields of the class (digit_at)
+        // straight from bit fields of the class and its carry word
+        // (digit_from_carries)
         const cplx *key = gkey + (size_t)st * l * 2 * M;
         // digits of level pair `pair`, their transforms, and the products with the
         // key rows of those levels added into A (in level order)
@@ -301,7 +302,8 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
             for (int i = 0; i < 8; i++)
 #pragma unroll
                 for (int c = 0; c < 2; c++)
-                    dg[c][i] = lev + c < l ? (double)digit_at((uint64_t)r[i], lev + c, lb, lq, tz[i]) : 0.0;
+                    dg[c][i] = lev + c < l ? (double)digit_from_carries((uint64_t)r[i], cv[i], lev + c, lb, lq)
+                                           : 0.0;
 #pragma unroll
             for (int c = 0; c < 2; c++)
 #pragma unroll

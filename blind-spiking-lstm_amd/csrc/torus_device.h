@@ -62,33 +62,37 @@ __device__ __forceinline__ int64_t residual_at(int64_t r, int S, int lq)
 }
 
 // The same digits without the recursion, read from bit fields of the centred
-// class u (lq bits, sign-extended).  up_at(S) is the round-half-even carry of
-// the residual at weight 2^S -- 1 iff residual_at(r, S) is the negative
+// class u (lq bits, sign-extended).  up(S) is the round-half-even carry of the
+// residual at weight 2^S -- 1 iff residual_at(r, S) is the negative
 // representative: at S = lq the sign of the class; S <= 0: 0; else bit S-1
 // set and (some lower bit set, or bit S set: the tie goes to even).  With
 // S = lq - j*lb and S1 = max(S - lb, 0), digit j of the recursion is
-//   d_j = bits [S1, S) of u + up_at(S1) - up_at(S) * 2^(S - S1)
-// (the residuals before and after level j differ by d_j * 2^S1).  tz is the
-// number of trailing zero bits of u (64 for u = 0).
-__device__ __forceinline__ int up_at(uint64_t u, int S, int lq, int tz)
+//   d_j = bits [S1, S) of u + up(S1) - up(S) * 2^(S - S1)
+// (the residuals before and after level j differ by d_j * 2^S1).
+//
+// carries_of(u) holds every carry at once: bit p is up(p + 1) for p < lq.
+// "Some bit below p" is "p above the lowest set bit of u", so
+//   bit p = u_p & (u_(p+1) | p > ctz(u)),
+// and bit lq-1 is the sign, which the formula gives through the sign-extended
+// bit lq except at lq = 64 (u = 2^63): or-ed in.  One word per coefficient
+// per level loop; each digit is then two fields and two carry bits.
+__device__ __forceinline__ uint64_t carries_of(uint64_t u, int lq)
 {
-    if (S >= lq)
-        return (int)(u >> (lq - 1)) & 1;
-    if (S <= 0)
-        return 0;
-    return (int)((u >> (S - 1)) & 1) & ((int)(tz < S - 1) | (int)((u >> S) & 1));
+    const uint64_t low = u & (0 - u);              // lowest set bit (0 for u = 0)
+    const uint64_t above = ~(low | (low - 1));     // positions above it
+    return (u & ((u >> 1) | above)) | (u & (1ULL << (lq - 1)));
 }
 
-__device__ __forceinline__ int trailing_zeros(uint64_t u) { return u ? __builtin_ctzll(u) : 64; }
-
-__device__ __forceinline__ int digit_at(uint64_t u, int j, int lb, int lq, int tz)
+__device__ __forceinline__ int digit_from_carries(uint64_t u, uint64_t cv, int j, int lb, int lq)
 {
     const int S = lq - j * lb;
     if (S <= 0)
         return 0;
     const int S1 = S > lb ? S - lb : 0;
     const int field = (int)((u >> S1) & ((1ULL << (S - S1)) - 1));
-    return field + up_at(u, S1, lq, tz) - (up_at(u, S, lq, tz) << (S - S1));
+    const int lo = S1 > 0 ? (int)((cv >> (S1 - 1)) & 1) : 0;
+    const int hi = (int)((cv >> (S - 1)) & 1);
+    return field + lo - (hi << (S - S1));
 }
 
 // Torus class of a rounded FP64 transform output (jax_inversefourier's

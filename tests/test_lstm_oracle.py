@@ -77,30 +77,31 @@ def test_digit_recursion_restarts_from_closed_form_residual(log_q, log_beta, l):
                 assert np.array_equal(d, D[j]), (k, j)
 
 
-def _digit_from_bits(x, j, log_beta, log_q):
-    """The closed form csrc/torus_device.h's digit_at computes for level j of
-    the float-rule recursion: with S = log_q - j*log_beta and S1 = max(S -
-    log_beta, 0), d_j = bits [S1, S) of x + up(S1) - up(S) * 2^(S - S1), where
-    up(S) is the round-half-even carry of the residual at weight 2^S: the sign
-    bit at S = log_q, 0 at S <= 0, else bit S-1 and (a lower bit set, or bit S)."""
+def _carries(x, log_q):
+    """csrc/torus_device.h's carries_of: bit p = up(p + 1), the round-half-even
+    carry of the residual at weight 2^(p+1): bit p of x and (bit p+1, or p
+    above the lowest set bit), the sign bit or-ed in at p = log_q - 1."""
     u = x.astype(np.uint64)
-    tz_low = lambda S: (u & np.uint64((1 << (S - 1)) - 1)) != 0
+    low = u & (np.uint64(0) - u)
+    above = ~(low | (low - np.uint64(1)))
+    return (u & ((u >> np.uint64(1)) | above)) | (u & np.uint64(1 << (log_q - 1)))
 
-    def up(S):
-        if S >= log_q:
-            return ((u >> np.uint64(log_q - 1)) & np.uint64(1)).astype(np.int64)
-        if S <= 0:
-            return np.zeros(u.shape, np.int64)
-        b = (u >> np.uint64(S - 1)) & np.uint64(1)
-        c = (u >> np.uint64(S)) & np.uint64(1)
-        return (b & (tz_low(S).astype(np.uint64) | c)).astype(np.int64)
 
+def _digit_from_bits(x, j, log_beta, log_q):
+    """The closed form csrc/torus_device.h's digit_from_carries computes for
+    level j of the float-rule recursion: with S = log_q - j*log_beta and S1 =
+    max(S - log_beta, 0), d_j = bits [S1, S) of x + up(S1) - up(S) *
+    2^(S - S1), the carries read from the carry word (bit S-1 is up(S))."""
+    u = x.astype(np.uint64)
+    cv = _carries(x, log_q)
     S = log_q - j * log_beta
     if S <= 0:
         return np.zeros(u.shape, np.int64)
     S1 = max(S - log_beta, 0)
     field = ((u >> np.uint64(S1)) & np.uint64((1 << (S - S1)) - 1)).astype(np.int64)
-    return field + up(S1) - (up(S) << (S - S1))
+    lo = ((cv >> np.uint64(S1 - 1)) & np.uint64(1)).astype(np.int64) if S1 > 0 else 0
+    hi = ((cv >> np.uint64(S - 1)) & np.uint64(1)).astype(np.int64)
+    return field + lo - (hi << (S - S1))
 
 
 @pytest.mark.parametrize("log_q,log_beta,l", [(64, 2, 20), (64, 13, 2), (64, 8, 8), (32, 7, 3),
@@ -112,7 +113,8 @@ def test_digits_from_bit_fields_equal_recursion(log_q, log_beta, l):
     to the oracle's digits at every level, with round-half ties planted at
     every level and the all-zero / sign-boundary classes included.  A carry
     that ignored the tie bit, the sticky lower bits, or the sign at the top
-    level fails here."""
+    level fails here, and so does a carry word that misses the sign at
+    log_q = 64 (the class -2^63 is planted)."""
     rng = np.random.default_rng(log_q * 1000 + log_beta * 10 + l)
     x = L.uniform(rng, 6000, log_q).astype(np.uint64)
     for j in range(l):
