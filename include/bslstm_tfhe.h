@@ -110,7 +110,10 @@ int bslstm_mul_bootstrap_rows(void *stream,
  * ciphertext is bound by reading the key slice from L2.
  *
  * Degree 2048 and l <= 3 only; at most (CU count / 2) ciphertexts (rounded
- * up to a multiple of 8), so that every workgroup is resident at once.
+ * up to a multiple of 8), so that every workgroup is resident at once.  Two
+ * pair-form launches that run concurrently (on different streams) share that
+ * budget: their ciphertexts together should not exceed it, or waits may run
+ * out (reported in the status word; never a hang).
  * workspace: device memory of bslstm_mul_bootstrap_pair_workspace_bytes()
  * bytes, cleared by the call itself (stream-ordered; one call at a time per
  * workspace).  Its first 4 bytes are a status word the caller may read after

@@ -160,3 +160,38 @@ def test_pair_rejects_outside_its_range(cuda_device, mult):
         mult.mul_bootstrap(dev(case.lut), dev(case.bsk_fft_k), None, idx, 7, 32, 1, form="pair")
     assert not mult.pair_applicable(2048, 1, big, 0)
     assert mult.pair_applicable(2048, 3, 8, 0)
+
+
+def test_pair_in_a_graph_replays_equal_eager(cuda_device, mult):
+    """The pair form captured in a HIP graph (its workspace cleared by a fill
+    kernel, not a memset): every replay, with eager work and synchronisations
+    in between and new inputs copied into the captured buffers, equals the
+    eager call on the same inputs."""
+    N, L_, c, B = 2048, 2, 3, 16
+    case = make_case(N=N, n=24, L=L_, log_beta=7, collapse=c, B=B, seed=71)
+    rng = np.random.default_rng(72)
+    bsk = dev(case.bsk_fft_k)
+    cts = [dev(rng.integers(-2 ** 31, 2 ** 31, size=(B, 2, N))) for _ in range(3)]
+    idxs = [dev(rng.integers(0, 2 * N, size=(B, 8, 8)).astype(np.int32)) for _ in range(3)]
+    want = [host(mult.mul_bootstrap(ct, bsk, None, ix, 7, 32, L_, form="pair"))
+            for ct, ix in zip(cts, idxs)]
+    ct_buf, idx_buf = cts[0].clone(), idxs[0].clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):                   # warm-up on the capture stream
+        mult.mul_bootstrap(ct_buf, bsk, None, idx_buf, 7, 32, L_, form="pair")
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = mult.mul_bootstrap(ct_buf, bsk, None, idx_buf, 7, 32, L_, form="pair")
+    for rep in range(6):
+        k = rep % 3
+        ct_buf.copy_(cts[k])
+        idx_buf.copy_(idxs[k])
+        g.replay()
+        got = host(out)
+        np.testing.assert_array_equal(got, want[k], err_msg=f"replay {rep}")
+        # eager work between replays, including another pair-form call
+        host(mult.mul_bootstrap(cts[(k + 1) % 3], bsk, None, idxs[(k + 1) % 3], 7, 32, L_,
+                                form="pair"))
