@@ -85,36 +85,58 @@ __global__ void rotate_extract_kernel(const int64_t *__restrict__ br, const doub
 // LWE key switch (schemas/LWE_jax.py:97-117, gadget_product_LWE :65-82)
 //
 // out[r, j] = sum_{lev,i} d[r,lev,i] * KSK[lev,i,j], j in [0, n_out]; column
-// n_out is the body key.  FP64 products and sums are exact for torus32
-// (|sum| < 2^53), as in the reference's float64 tensordot.  Each workgroup
-// owns RB ciphertexts x 256 output columns; digits are staged in LDS and the
-// key streams coalesced along j.
+// n_out is the body key.  The digits are the reference's float rule on the
+// input as given (x = a/q, no centring: polynomial_jax.py:90-111).  FP64
+// products and sums are exact for torus32 (|sum| < 2^53), as in the
+// reference's float64 tensordot, so partial sums over any split of (lev, i)
+// add up exactly in any order: each workgroup owns RB ciphertexts x 256
+// output columns x one range of input coefficients (the K split that fills
+// the device), stages its digits in LDS, streams the key coalesced along j
+// and adds its partial sums into the zeroed outputs with FP64 atomics; a
+// final pass centres them (key_switch_final_kernel).  Deterministic: every
+// partial sum is an integer below 2^53.
 // ---------------------------------------------------------------------------
 constexpr int KS_RB = 32;          // ciphertexts per workgroup
 constexpr int KS_LDS_DOUBLES = 8192; // digit tile: l_ks * KS_RB * ic <= 64 KiB
 constexpr int KS_LMAX = 64;        // max decomposition levels
 constexpr int KS_THREADS = 256;
+constexpr int KS_UNROLL = 4;       // key loads in flight per thread
+
+__global__ void ks_zero_kernel(double *__restrict__ a_out, double *__restrict__ b_out, int64_t na,
+                               int64_t nb)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < na + nb;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        if (e < na)
+            a_out[e] = 0.0;
+        else
+            b_out[e - na] = 0.0;
+    }
+}
 
 __global__ __launch_bounds__(KS_THREADS) void key_switch_kernel(
-    const double *__restrict__ a_in, const double *__restrict__ b_in, const double *__restrict__ ksk_a,
+    const double *__restrict__ a_in, const double *__restrict__ ksk_a,
     const double *__restrict__ ksk_b, int batch, int n_in, int n_out, double beta, int l_ks, int ic_max,
-    double q, double inv_q, double *__restrict__ a_out, double *__restrict__ b_out)
+    int i_per_split, double q, double *__restrict__ a_out, double *__restrict__ b_out)
 {
     extern __shared__ double dig[];   // [l_ks][KS_RB][ic_max]
     const int j = blockIdx.x * KS_THREADS + threadIdx.x;
     const int r0 = blockIdx.y * KS_RB;
+    const int i_begin = blockIdx.z * i_per_split;
+    const int i_end = min(n_in, i_begin + i_per_split);
     const bool active = j <= n_out;
     double acc[KS_RB];
 #pragma unroll
     for (int r = 0; r < KS_RB; r++)
         acc[r] = 0.0;
 
-    for (int i0 = 0; i0 < n_in; i0 += ic_max) {
+    for (int i0 = i_begin; i0 < i_end; i0 += ic_max) {
+        const int ic = min(ic_max, i_end - i0);
         __syncthreads();
         for (int e = threadIdx.x; e < KS_RB * ic_max; e += KS_THREADS) {
             const int r = e / ic_max, ii = e % ic_max;
             const int row = r0 + r, i = i0 + ii;
-            double x = (row < batch && i < n_in) ? a_in[(size_t)row * n_in + i] / q : 0.0;
+            double x = (row < batch && ii < ic) ? a_in[(size_t)row * n_in + i] / q : 0.0;
             for (int lev = 0; lev < l_ks; lev++) {
                 const double xb = x * beta;
                 const double d = rint(xb);
@@ -124,15 +146,23 @@ __global__ __launch_bounds__(KS_THREADS) void key_switch_kernel(
         }
         __syncthreads();
         if (active) {
-            const int ic = min(ic_max, n_in - i0);
+            const double *kcol = j < n_out ? ksk_a + j : ksk_b;
+            const size_t kstride = j < n_out ? (size_t)n_out : 1;
             for (int lev = 0; lev < l_ks; lev++) {
-                for (int ii = 0; ii < ic; ii++) {
-                    const size_t kidx = (size_t)lev * n_in + i0 + ii;
-                    const double kv = j < n_out ? ksk_a[kidx * n_out + j] : ksk_b[kidx];
-                    const double *dr = dig + (lev * KS_RB) * ic_max + ii;
+                const double *krow = kcol + ((size_t)lev * n_in + i0) * kstride;
+                const double *drow = dig + (lev * KS_RB) * ic_max;
+                // KS_UNROLL independent key loads in flight; digits past ic
+                // are zero in LDS, so the tail adds zeros (exact)
+                for (int ii = 0; ii < ic; ii += KS_UNROLL) {
+                    double kv[KS_UNROLL];
 #pragma unroll
-                    for (int r = 0; r < KS_RB; r++)
-                        acc[r] = fma(dr[r * ic_max], kv, acc[r]);
+                    for (int u = 0; u < KS_UNROLL; u++)
+                        kv[u] = ii + u < ic ? krow[(size_t)(ii + u) * kstride] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < KS_UNROLL; u++)
+#pragma unroll
+                        for (int r = 0; r < KS_RB; r++)
+                            acc[r] = fma(drow[r * ic_max + ii + u], kv[u], acc[r]);
                 }
             }
         }
@@ -144,11 +174,28 @@ __global__ __launch_bounds__(KS_THREADS) void key_switch_kernel(
         const int row = r0 + r;
         if (row >= batch)
             break;
-        double v = acc[r] - q * rint(acc[r] * inv_q);
-        if (j < n_out) {
-            a_out[(size_t)row * n_out + j] = v;
+        if (j < n_out)
+            atomicAdd(&a_out[(size_t)row * n_out + j], acc[r]);
+        else
+            atomicAdd(&b_out[row], acc[r]);
+    }
+}
+
+// centred mod q of the exact sums; the body adds b first (LWE_jax.py:117)
+__global__ void key_switch_final_kernel(const double *__restrict__ b_in, int batch, int n_out,
+                                        double q, double inv_q, double *__restrict__ a_out,
+                                        double *__restrict__ b_out)
+{
+    const int64_t na = (int64_t)batch * n_out;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < na + batch;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        if (e < na) {
+            const double v = a_out[e];
+            a_out[e] = v - q * rint(v * inv_q);
         } else {
-            double bb = b_in[row] + v;
+            const int64_t row = e - na;
+            const double v = b_out[row] - q * rint(b_out[row] * inv_q);
+            const double bb = b_in[row] + v;
             b_out[row] = bb - q * rint(bb * inv_q);
         }
     }
@@ -397,14 +444,45 @@ extern "C" int bslstm_key_switch_lwe(void *stream, const double *a_in, const dou
         return BSLSTM_OK;
     if (!a_in || !b_in || !ksk_a || !ksk_b || !a_out || !b_out)
         return fail(BSLSTM_INVALID_ARGUMENT, "NULL operand");
+    // the outputs are zeroed first and accumulated into: they must not
+    // overlap the inputs
+    {
+        auto overlap = [](const void *p, int64_t np, const void *o, int64_t no) {
+            const char *a = (const char *)p, *b = (const char *)o;
+            return a < b + no && b < a + np;
+        };
+        const int64_t bytes_a_out = batch * n_out * 8, bytes_b = batch * 8;
+        if (overlap(a_in, batch * n_in * 8, a_out, bytes_a_out) ||
+            overlap(a_in, batch * n_in * 8, b_out, bytes_b) ||
+            overlap(b_in, bytes_b, a_out, bytes_a_out) || overlap(b_in, bytes_b, b_out, bytes_b))
+            return fail(BSLSTM_INVALID_ARGUMENT, "key-switch outputs overlap its inputs");
+    }
     const double q = ldexp(1.0, (int)log2_q);
-    dim3 grid((unsigned)((n_out + 1 + KS_THREADS - 1) / KS_THREADS),
-              (unsigned)((batch + KS_RB - 1) / KS_RB));
-    const int ic_max = (int)std::min<int64_t>(32, KS_LDS_DOUBLES / (l_ks * KS_RB));
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t na = batch * n_out;
+    hipLaunchKernelGGL(ks_zero_kernel, dim3((unsigned)std::min<int64_t>(1024, (na + batch + 255) / 256)),
+                       dim3(256), 0, s, a_out, b_out, na, batch);
+    const unsigned col_tiles = (unsigned)((n_out + 1 + KS_THREADS - 1) / KS_THREADS);
+    const unsigned row_tiles = (unsigned)((batch + KS_RB - 1) / KS_RB);
+    // staging width: a multiple of the unroll (the digits past a short last
+    // chunk are zero in LDS, and every read stays inside the tile); >= 4
+    // since l_ks <= KS_LMAX
+    const int ic_max = (int)(std::min<int64_t>(32, KS_LDS_DOUBLES / (l_ks * KS_RB)) / KS_UNROLL *
+                             KS_UNROLL);
+    // split the input coefficients so that the launch has ~2048 workgroups
+    // (8 per CU), each range a multiple of the staging width
+    const int64_t tiles = (int64_t)col_tiles * row_tiles;
+    const int64_t max_splits = (n_in + ic_max - 1) / ic_max;
+    const int64_t splits = std::max<int64_t>(1, std::min<int64_t>(max_splits, (2048 + tiles - 1) / tiles));
+    const int64_t chunks = (max_splits + splits - 1) / splits;
+    const int i_per_split = (int)(chunks * ic_max);
+    const unsigned zsplits = (unsigned)((n_in + i_per_split - 1) / i_per_split);
     const size_t lds = sizeof(double) * (size_t)l_ks * KS_RB * ic_max;
-    hipLaunchKernelGGL(key_switch_kernel, grid, dim3(KS_THREADS), lds, (hipStream_t)stream, a_in,
-                       b_in, ksk_a, ksk_b, (int)batch, (int)n_in, (int)n_out,
-                       ldexp(1.0, (int)log2_beta_ks), (int)l_ks, ic_max, q, 1.0 / q, a_out, b_out);
+    hipLaunchKernelGGL(key_switch_kernel, dim3(col_tiles, row_tiles, zsplits), dim3(KS_THREADS), lds,
+                       s, a_in, ksk_a, ksk_b, (int)batch, (int)n_in, (int)n_out,
+                       ldexp(1.0, (int)log2_beta_ks), (int)l_ks, ic_max, i_per_split, q, a_out, b_out);
+    hipLaunchKernelGGL(key_switch_final_kernel, dim3((unsigned)std::min<int64_t>(1024, (na + batch + 255) / 256)),
+                       dim3(256), 0, s, b_in, (int)batch, (int)n_out, q, 1.0 / q, a_out, b_out);
     return launch_status("key switch");
 }
 

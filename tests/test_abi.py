@@ -114,6 +114,12 @@ def test_lstm_path_entry_points_reject_on_host():
     assert "form must be" in lib.bslstm_last_error().decode()
     # aliasing the key-switch output onto its input is refused
     assert lib.bslstm_key_switch_lwe_i64(None, 8, 16, 8, 8, 4, 2048, 702, 1, 20, 64, 8, 16) == bad
+    assert lib.bslstm_key_switch_lwe(None, 4096, 1 << 20, 8, 8, 4, 1024, 630, 4, 5, 32,
+                                     4096 + 8 * 1024, 1 << 21) == bad
+    assert "overlap" in lib.bslstm_last_error().decode()
+    assert lib.bslstm_key_switch_lwe(None, 4096, 1 << 20, 8, 8, 4, 1024, 630, 4, 5, 32, 1 << 22,
+                                     1 << 20) == bad
+    assert "overlap" in lib.bslstm_last_error().decode()
     assert _core.lib().bslstm_packing_workspace_bytes(2, 2048, 2048) == \
         2 * 2048 * 2048 * 8 + 2 * 256 * 2 * 1024 * 16
 

@@ -178,6 +178,9 @@ def test_rotate_extract_bit_exact(cuda_device):
 
 
 def test_key_switch_bit_exact(cuda_device):
+    """The FP64 key switch (K split over workgroups, exact FP64 atomics)
+    equals the oracle's float-rule key switch; 70 ciphertexts and three
+    shapes, so the split, the row tiles and a short last chunk all occur."""
     from schemas.LWE_jax import key_switch_LWE
     rng = np.random.default_rng(4)
     for (n_in, n_out, lb, l) in [(1024, 630, 4, 5), (2048, 750, 4, 5), (300, 37, 2, 12)]:
@@ -186,6 +189,9 @@ def test_key_switch_bit_exact(cuda_device):
         ka, kb = O.keyswitch_key(rng, s_new, s_old, 32, lb, l, 2.0 ** 12)
         a = rng.integers(-2 ** 31, 2 ** 31, size=(70, n_in))
         a[0, :3] = [2 ** 31 - 1, -2 ** 31, 2 ** (32 - lb - 1)]     # extremes and a tie
+        # representations outside [-q/2, q/2): the float rule decomposes
+        # a/q as given (a first digit of +beta/2 at +q/2, larger beyond)
+        a[1, :4] = [2 ** 31, 3 * 2 ** 30, -3 * 2 ** 30, 2 ** 32 - 1]
         b = rng.integers(-2 ** 31, 2 ** 31, size=70)
         ra, rb = O.key_switch_lwe(ka, kb, a, b, 32, lb, l)
         params = {"beta_ks": 2 ** lb, "l_ks": l, "q": 2.0 ** 32}
