@@ -182,8 +182,8 @@ __global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
     const int line_end = min(lines, line0 + lpw);
     // warm-up loads per thread per step: the whole half slice (4096 lines at
     // l = 2, 6144 at l = 3, m = 8) from two workgroups per row and XCD on;
-    // a workgroup warming alone takes four lines per thread (half the slice:
-    // measured faster than all of it)
+    // a workgroup warming alone takes four lines per thread (2048 lines, part
+    // of the slice: measured faster than warming all of it)
     constexpr int PF = L <= 2 ? 8 : 6;
     const int npf = P == 1 ? 4 : PF;
     uint32_t warm = 0, pf[PF] = {};
