@@ -1,5 +1,5 @@
 // Latency form of the collapsed blind rotation over two workgroups per
-// ciphertext (the "pair" form), N = 2048.  Same semantics and bit-identical
+// ciphertext (the "pair" form), N = 1024 and 2048.  Same semantics and bit-identical
 // outputs as the one-workgroup forms (blind_rotate.hip; the reference kernel
 // csrc/mul_bootstrap.cuh:20-136).
 //
@@ -40,7 +40,9 @@
 
 namespace bslstm {
 
-constexpr int PAIR_N = 2048;
+// degrees of the pair form: one 512-thread workgroup per CU at N = 2048
+// (160 KiB of LDS at l = 3), two 256-thread ones per CU at N = 1024
+static bool pair_degree(int64_t n) { return n == 1024 || n == 2048; }
 
 // Workspace (device memory the caller passes): [0] status word, flags, then
 // the exchange buffers (double-buffered by step parity).
@@ -53,12 +55,12 @@ struct PairWs {
 
 static int64_t pair_count(int64_t batch) { return (batch + 7) / 8 * 8; }
 
-static int64_t pair_ws_bytes(int64_t batch, int64_t l)
+static int64_t pair_ws_bytes(int64_t batch, int64_t l, int64_t degree)
 {
     const int64_t P = pair_count(batch);
     const int64_t head = (1 + 4 * P) * 4;
     const int64_t head_al = (head + 255) / 256 * 256;
-    return head_al + P * 2 * 2 * l * (PAIR_N / 2) * 16;
+    return head_al + P * 2 * 2 * l * (degree / 2) * 16;
 }
 
 static PairWs pair_ws(void *ws, int64_t batch)
@@ -94,16 +96,16 @@ __device__ __forceinline__ bool wait_at_least(const uint32_t *f, uint32_t target
     return __hip_atomic_load(f, __ATOMIC_RELAXED, SCOPE) >= target;
 }
 
-template <int L, int ACC, int MP>
-__global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
+template <int N, int L, int ACC, int MP>
+__global__ __launch_bounds__(N / 4, 2) void br_pair_kernel(
     const int64_t *__restrict__ ct_init, int64_t ct_stride, const cplx *__restrict__ bsk,
     const int32_t *__restrict__ indices, int64_t *__restrict__ out, int64_t batch, int n_iter,
     int m_arg, int log2_beta, int log2_q, KeySlots ks, PairWs ws, int spin_limit)
 {
-    constexpr int N = PAIR_N;
     const int m = MP > 0 ? MP : m_arg;
     constexpr int M = Fold<N>::M, T = Fold<N>::T;
-    // both rows' L spectra next to the full-period root table: 160 KiB at l = 3
+    // both rows' L spectra next to the full-period root table: 160 KiB at
+    // N = 2048, l = 3 (80 KiB at N = 1024)
     static_assert(L <= 3, "the pair form holds l <= 3");
     constexpr bool FULL = true;
     __shared__ cplx own[L * M];                   // this row's digit spectra, then a product copy
@@ -180,8 +182,9 @@ __global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
     const int lpw = (lines + P - 1) / P;
     const int line0 = ((b >> 4) % P) * lpw;
     const int line_end = min(lines, line0 + lpw);
-    // warm-up loads per thread per step: the whole half slice (4096 lines at
-    // l = 2, 6144 at l = 3, m = 8) from two workgroups per row and XCD on;
+    // warm-up loads per thread per step: the whole half slice (at N = 2048,
+    // m = 8: 4096 lines at l = 2, 6144 at l = 3) from two workgroups per row
+    // and XCD on;
     // a workgroup warming alone takes four lines per thread (2048 lines, part
     // of the slice: measured faster than warming all of it)
     constexpr int PF = L <= 2 ? 8 : 6;
@@ -393,35 +396,51 @@ __global__ __launch_bounds__(PAIR_N / 4, 2) void br_pair_kernel(
     }
 }
 
-template <int L, int MP>
+template <int N, int L, int MP>
 static void launch_pair_acc(hipStream_t s, const int64_t *ct, int64_t st, const cplx *bk,
                             const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m,
                             int lb, int lq, KeySlots ks, PairWs ws, int spin)
 {
-    const dim3 grid((unsigned)(2 * pair_count(batch))), block(PAIR_N / 4);
+    const dim3 grid((unsigned)(2 * pair_count(batch))), block(N / 4);
     if (lq <= 32)
-        hipLaunchKernelGGL((br_pair_kernel<L, ACC_32, MP>), grid, block, 0, s, ct, st, bk, idx, out,
-                           batch, n_iter, m, lb, lq, ks, ws, spin);
+        hipLaunchKernelGGL((br_pair_kernel<N, L, ACC_32, MP>), grid, block, 0, s, ct, st, bk, idx,
+                           out, batch, n_iter, m, lb, lq, ks, ws, spin);
     else if (L * lb + 1 <= 32)
-        hipLaunchKernelGGL((br_pair_kernel<L, ACC_HI, MP>), grid, block, 0, s, ct, st, bk, idx, out,
-                           batch, n_iter, m, lb, lq, ks, ws, spin);
+        hipLaunchKernelGGL((br_pair_kernel<N, L, ACC_HI, MP>), grid, block, 0, s, ct, st, bk, idx,
+                           out, batch, n_iter, m, lb, lq, ks, ws, spin);
     else
-        hipLaunchKernelGGL((br_pair_kernel<L, ACC_64, MP>), grid, block, 0, s, ct, st, bk, idx, out,
-                           batch, n_iter, m, lb, lq, ks, ws, spin);
+        hipLaunchKernelGGL((br_pair_kernel<N, L, ACC_64, MP>), grid, block, 0, s, ct, st, bk, idx,
+                           out, batch, n_iter, m, lb, lq, ks, ws, spin);
 }
 
-template <int L>
+template <int N, int L>
 static void launch_pair_l(hipStream_t s, const int64_t *ct, int64_t st, const cplx *bk,
                           const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m, int lb,
                           int lq, KeySlots ks, PairWs ws, int spin)
 {
     if (m == 8)
-        launch_pair_acc<L, 8>(s, ct, st, bk, idx, out, batch, n_iter, m, lb, lq, ks, ws, spin);
+        launch_pair_acc<N, L, 8>(s, ct, st, bk, idx, out, batch, n_iter, m, lb, lq, ks, ws, spin);
     else
-        launch_pair_acc<L, 0>(s, ct, st, bk, idx, out, batch, n_iter, m, lb, lq, ks, ws, spin);
+        launch_pair_acc<N, L, 0>(s, ct, st, bk, idx, out, batch, n_iter, m, lb, lq, ks, ws, spin);
 }
 
-static int max_pairs_resident()
+template <int N>
+static void launch_pair_n(hipStream_t s, const int64_t *ct, int64_t st, const cplx *bk,
+                          const int32_t *idx, int64_t *out, int64_t batch, int n_iter, int m, int lb,
+                          int lq, int l, KeySlots ks, PairWs ws, int spin)
+{
+    switch (l) {
+    case 1: launch_pair_l<N, 1>(s, ct, st, bk, idx, out, batch, n_iter, m, lb, lq, ks, ws, spin); break;
+    case 2: launch_pair_l<N, 2>(s, ct, st, bk, idx, out, batch, n_iter, m, lb, lq, ks, ws, spin); break;
+    default: launch_pair_l<N, 3>(s, ct, st, bk, idx, out, batch, n_iter, m, lb, lq, ks, ws, spin); break;
+    }
+}
+
+// Ciphertexts the device holds with every workgroup resident: two
+// workgroups per ciphertext; at N = 2048 one per CU (8 waves, up to 160 KiB
+// of LDS), at N = 1024 two per CU (4 waves and <= 80 KiB each, within the
+// two waves per SIMD the launch bounds give).
+static int max_pairs_resident(int64_t degree)
 {
     static int cus[64] = {0};
     int dev = 0;
@@ -433,7 +452,7 @@ static int max_pairs_resident()
             return 0;
         cus[dev] = n;
     }
-    return cus[dev] / 2;    // one workgroup per CU (LDS), two per ciphertext
+    return degree == 1024 ? cus[dev] : cus[dev] / 2;
 }
 
 static int run_pair(void *stream, const int64_t *ct_init, int64_t ct_elems, const double *bsk,
@@ -449,18 +468,18 @@ static int run_pair(void *stream, const int64_t *ct_init, int64_t ct_elems, cons
                                      degree, ct_init, &st);
     if (rc != BSLSTM_OK)
         return rc;
-    if (degree != PAIR_N)
-        return fail(BSLSTM_INVALID_ARGUMENT, "the pair form runs at degree 2048 only");
+    if (!pair_degree(degree))
+        return fail(BSLSTM_INVALID_ARGUMENT, "the pair form runs at degree 1024 or 2048");
     if (l > 3)
         return fail(BSLSTM_INVALID_ARGUMENT, "the pair form holds l <= 3");
     if (spin_limit < 1 || spin_limit > (1LL << 30))
         return fail(BSLSTM_INVALID_ARGUMENT, "spin_limit out of range [1, 2^30]");
     if (batch == 0)
         return BSLSTM_OK;
-    if (!workspace || workspace_bytes < pair_ws_bytes(batch, l))
+    if (!workspace || workspace_bytes < pair_ws_bytes(batch, l, degree))
         return fail(BSLSTM_INVALID_ARGUMENT, "workspace of %lld bytes; need %lld",
-                    (long long)workspace_bytes, (long long)pair_ws_bytes(batch, l));
-    const int maxp = max_pairs_resident();
+                    (long long)workspace_bytes, (long long)pair_ws_bytes(batch, l, degree));
+    const int maxp = max_pairs_resident(degree);
     if (pair_count(batch) > maxp)
         return fail(BSLSTM_INVALID_ARGUMENT,
                     "the pair form holds at most %d ciphertexts on this device (batch %lld)", maxp,
@@ -472,11 +491,12 @@ static int run_pair(void *stream, const int64_t *ct_init, int64_t ct_elems, cons
     const cplx *bk = reinterpret_cast<const cplx *>(bsk);
     const int ni = (int)bsk_dims[0], mm = (int)bsk_dims[4], lb = (int)log2_beta, lq = (int)log2_q;
     const int spin = (int)spin_limit;
-    switch (l) {
-    case 1: launch_pair_l<1>(s, ct_init, st, bk, indices, output, batch, ni, mm, lb, lq, ks, ws, spin); break;
-    case 2: launch_pair_l<2>(s, ct_init, st, bk, indices, output, batch, ni, mm, lb, lq, ks, ws, spin); break;
-    default: launch_pair_l<3>(s, ct_init, st, bk, indices, output, batch, ni, mm, lb, lq, ks, ws, spin); break;
-    }
+    if (degree == 1024)
+        launch_pair_n<1024>(s, ct_init, st, bk, indices, output, batch, ni, mm, lb, lq, (int)l, ks,
+                            ws, spin);
+    else
+        launch_pair_n<2048>(s, ct_init, st, bk, indices, output, batch, ni, mm, lb, lq, (int)l, ks,
+                            ws, spin);
     return launch_status("pair blind rotation");
 }
 
@@ -485,9 +505,9 @@ static int run_pair(void *stream, const int64_t *ct_init, int64_t ct_elems, cons
 extern "C" int64_t bslstm_mul_bootstrap_pair_workspace_bytes(int64_t batch, int64_t l,
                                                              int64_t degree)
 {
-    if (batch < 0 || l < 1 || l > 3 || degree != bslstm::PAIR_N)
+    if (batch < 0 || l < 1 || l > 3 || !bslstm::pair_degree(degree))
         return -1;
-    return bslstm::pair_ws_bytes(batch, l);
+    return bslstm::pair_ws_bytes(batch, l, degree);
 }
 
 extern "C" int bslstm_mul_bootstrap_pair(void *stream, const int64_t *ct_init, int64_t ct_elems,

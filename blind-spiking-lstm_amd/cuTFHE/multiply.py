@@ -65,12 +65,13 @@ _PAIR_LAST: dict = {}      # device index -> workspace of the last pair-form cal
 
 
 def pair_applicable(degree: int, l: int, batch: int, device=None) -> bool:
-    """Whether the pair form takes this shape: degree 2048, l <= 3 and every
-    workgroup resident at once (two CUs per ciphertext, batch rounded up to 8)."""
-    if degree != 2048 or not 1 <= l <= 3 or batch < 1:
+    """Whether the pair form takes this shape: degree 1024 or 2048, l <= 3 and
+    every workgroup resident at once (two workgroups per ciphertext, batch
+    rounded up to 8; one workgroup per CU at N = 2048, two at N = 1024)."""
+    if degree not in (1024, 2048) or not 1 <= l <= 3 or batch < 1:
         return False
     cus = torch.cuda.get_device_properties(device).multi_processor_count
-    return (batch + 7) // 8 * 8 <= cus // 2
+    return (batch + 7) // 8 * 8 <= (cus if degree == 1024 else cus // 2)
 
 
 def pair_status(device=None) -> int:
@@ -87,7 +88,7 @@ def _pair_call(ct, bsk, arr, rot, idx, out, batch, log_beta, log_q, l, degree, s
     ws_bytes = int(_core.lib().bslstm_mul_bootstrap_pair_workspace_bytes(batch, l, degree))
     if ws_bytes < 0:
         raise _core.TFHEInvalidArgument(
-            f"the pair form runs at degree 2048 and l <= 3 (got N={degree}, l={l})")
+            f"the pair form runs at degree 1024 or 2048 and l <= 3 (got N={degree}, l={l})")
     ws = torch.empty(max(ws_bytes, 4), dtype=torch.int8, device=ct.device)
     spin = PAIR_SPIN_LIMIT if spin_limit is None else int(spin_limit)
     if slots is None:
@@ -216,10 +217,11 @@ def mul_bootstrap(ct_init: torch.Tensor, bsk_reshaped: torch.Tensor, all_rot_fft
     shape); "batched": two ciphertexts per workgroup sharing every key load;
     "rows": one ciphertext per workgroup with its two accumulator rows on two
     thread groups (the latency form for small batches);
-    "pair": two workgroups (CUs) per ciphertext, one accumulator row each,
-    exchanging the digit spectra once per step (degree 2048, l <= 3, batch up
-    to CUs/2; the lower-latency form for a few dozen ciphertexts; spin_limit
-    bounds each wait on the partner, see pair_status);
+    "pair": two workgroups per ciphertext, one accumulator row each,
+    exchanging the digit spectra once per step (degree 1024 or 2048, l <= 3,
+    batch up to CUs/2 at N = 2048 and CUs at N = 1024; the lower-latency form
+    for up to a few hundred ciphertexts; spin_limit bounds each wait on the
+    partner, see pair_status);
     "split": exact integer products as int8 MFMA GEMMs (log_q <= 32; the key
     is prepared once per key tensor and cached), equal to the exact product;
     "auto": the autotuner's choice for this shape (timed on first use; an

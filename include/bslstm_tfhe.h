@@ -109,8 +109,9 @@ int bslstm_mul_bootstrap_rows(void *stream,
  * the step's key slice: for a few dozen ciphertexts, where one CU per
  * ciphertext is bound by reading the key slice from L2.
  *
- * Degree 2048 and l <= 3 only; at most (CU count / 2) ciphertexts (rounded
- * up to a multiple of 8), so that every workgroup is resident at once.  Two
+ * Degree 1024 or 2048 and l <= 3; at most CU count / 2 ciphertexts at
+ * N = 2048 (one workgroup per CU) and CU count at N = 1024 (two per CU),
+ * rounded up to a multiple of 8, so that every workgroup is resident at once.  Two
  * pair-form launches that run concurrently (on different streams) share that
  * budget: their ciphertexts together should not exceed it, or waits may run
  * out (reported in the status word; never a hang).

@@ -163,7 +163,8 @@ def test_pair_form_entry_points_reject_on_host():
     ws = lib.bslstm_mul_bootstrap_pair_workspace_bytes(B, 3, 2048)
     assert ws == 256 + 8 * 2 * 2 * 3 * 1024 * 16
     assert lib.bslstm_mul_bootstrap_pair_workspace_bytes(B, 4, 2048) == -1
-    assert lib.bslstm_mul_bootstrap_pair_workspace_bytes(B, 3, 1024) == -1
+    assert lib.bslstm_mul_bootstrap_pair_workspace_bytes(B, 3, 1024) == 256 + 8 * 2 * 2 * 3 * 512 * 16
+    assert lib.bslstm_mul_bootstrap_pair_workspace_bytes(B, 3, 512) == -1
 
     def call(**kw):
         a = dict(ct_elems=4096, idx_elems=B * 250 * 8, batch=B, lb=7, lq=32, l=3, degree=2048,
@@ -172,8 +173,8 @@ def test_pair_form_entry_points_reject_on_host():
         return lib.bslstm_mul_bootstrap_pair(None, 8, a["ct_elems"], 8, a["dims"], None, 0, 0, 8,
                                              a["idx_elems"], 8, a["batch"], a["lb"], a["lq"],
                                              a["l"], a["degree"], 8, a["ws_bytes"], a["spin"])
-    assert call(degree=1024, ct_elems=2048, dims=(_core._I64 * 6)(250, 2, 3, 2, 8, 512)) == bad
-    assert "degree 2048" in lib.bslstm_last_error().decode()
+    assert call(degree=512, ct_elems=1024, dims=(_core._I64 * 6)(250, 2, 3, 2, 8, 256)) == bad
+    assert "degree 1024 or 2048" in lib.bslstm_last_error().decode()
     assert call(l=4, dims=(_core._I64 * 6)(250, 2, 4, 2, 8, 1024)) == bad
     assert "l <= 3" in lib.bslstm_last_error().decode()
     assert call(spin=0) == bad and "spin_limit" in lib.bslstm_last_error().decode()

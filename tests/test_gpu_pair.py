@@ -36,17 +36,20 @@ def mult():
     return multiply
 
 
-def _max_batch():
-    return torch.cuda.get_device_properties(0).multi_processor_count // 2 // 8 * 8
+def _max_batch(N=2048):
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    return (cus if N == 1024 else cus // 2) // 8 * 8
 
 
-@pytest.mark.parametrize("L_,c,B", [(3, 3, 1), (3, 3, 9), (2, 2, 40), (1, 3, 17), (3, 3, -1)])
-def test_pair_equals_one_workgroup_forms_torus32(cuda_device, mult, L_, c, B):
+@pytest.mark.parametrize("N,L_,c,B", [(2048, 3, 3, 1), (2048, 3, 3, 9), (2048, 2, 2, 40),
+                                      (2048, 1, 3, 17), (2048, 3, 3, -1), (1024, 3, 3, 1),
+                                      (1024, 3, 3, 57), (1024, 2, 3, 130), (1024, 3, 3, -1)])
+def test_pair_equals_one_workgroup_forms_torus32(cuda_device, mult, N, L_, c, B):
     """log2 q = 32 (32-bit accumulator), per-ciphertext test polynomials,
     ragged batches (the last group of 8 pairs part-filled) and the largest
-    batch the device holds (B = -1)."""
-    B = _max_batch() if B < 0 else B
-    N = 2048
+    batch the device holds (B = -1: two workgroups per CU at N = 1024, one
+    at N = 2048)."""
+    B = _max_batch(N) if B < 0 else B
     case = make_case(N=N, n=12 * c, L=L_, log_beta=7, collapse=c, B=B, seed=7 * B + L_)
     rng = np.random.default_rng(B + c)
     cts = dev(rng.integers(-2 ** 31, 2 ** 31, size=(B, 2, N)))
@@ -55,7 +58,7 @@ def test_pair_equals_one_workgroup_forms_torus32(cuda_device, mult, L_, c, B):
     r_rows = host(mult.mul_bootstrap(cts, bsk, None, idx, 7, 32, L_, form="rows"))
     r_pair = host(mult.mul_bootstrap(cts, bsk, None, idx, 7, 32, L_, form="pair"))
     status = mult.pair_status()
-    print(f"B={B} L={L_} c={c}: status word {status:#x}")
+    print(f"N={N} B={B} L={L_} c={c}: status word {status:#x}")
     assert status & 1 == 0
     np.testing.assert_array_equal(r_pair, r_rows)
     if B <= 9:
@@ -148,10 +151,10 @@ def test_pair_tiny_wait_bound_ends_and_next_call_is_correct(cuda_device, mult):
 
 
 def test_pair_rejects_outside_its_range(cuda_device, mult):
-    case = make_case(N=1024, n=24, L=3, log_beta=7, collapse=3, B=2, seed=61)
-    idx = dev(O.rotation_indices(case.a, 32, 1024, 3, 1))
+    case = make_case(N=512, n=24, L=3, log_beta=7, collapse=3, B=2, seed=61)
+    idx = dev(O.rotation_indices(case.a, 32, 512, 3, 1))
     from cuTFHE import _core
-    with pytest.raises(_core.TFHEInvalidArgument, match="degree 2048"):
+    with pytest.raises(_core.TFHEInvalidArgument, match="degree 1024 or 2048"):
         mult.mul_bootstrap(dev(case.lut), dev(case.bsk_fft_k), None, idx, 7, 32, 3, form="pair")
     big = _max_batch() + 8
     case = make_case(N=2048, n=6, L=1, log_beta=7, collapse=3, B=big, seed=62)
@@ -160,6 +163,9 @@ def test_pair_rejects_outside_its_range(cuda_device, mult):
         mult.mul_bootstrap(dev(case.lut), dev(case.bsk_fft_k), None, idx, 7, 32, 1, form="pair")
     assert not mult.pair_applicable(2048, 1, big, 0)
     assert mult.pair_applicable(2048, 3, 8, 0)
+    assert mult.pair_applicable(1024, 3, _max_batch(1024), 0)
+    assert not mult.pair_applicable(1024, 3, _max_batch(1024) + 8, 0)
+    assert not mult.pair_applicable(512, 3, 8, 0)
 
 
 def test_pair_in_a_graph_replays_equal_eager(cuda_device, mult):
