@@ -229,7 +229,8 @@ def run_gpu(args):
 
 
 def run_form_sweep():
-    """Blind-rotation time of the three launch forms at several batch sizes, at
+    """Blind-rotation time of the launch forms at several batch sizes (the pair
+    form where its co-residency limit admits the batch), at
     the batched-PBS parameters (torus32, l=3) and at the LSTM's (q=2^64, l=2,
     Bg=2^13, n=702): the data the autotuner's choices rest on.  Random keys and
     inputs (the time does not depend on their values); 3-launch minimum."""
@@ -1046,7 +1047,7 @@ def main():
     ap.add_argument("--pbs-keyswitch", type=int, default=1,
                     help="run the 256-ciphertext N=1024 PBS + key switch case (0: skip)")
     ap.add_argument("--form-sweep", type=int, default=1,
-                    help="time the three blind-rotation forms at several batch sizes (0: skip)")
+                    help="time the blind-rotation launch forms at several batch sizes (0: skip)")
     ap.add_argument("--split", type=int, default=1,
                     help="also time the split form (exact int8 MFMA) at the headline workload")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)

@@ -418,7 +418,8 @@ def test_text_rotation_and_extraction_on_hip_kernels(cuda_device, dtype):
 def test_pbs_config2_full_batch(cuda_device, mult):
     """The batched-PBS workload at its full size: 16384 ciphertexts, n=750,
     N=2048, l=3, log2 Bg=7, 4 LUTs, 4-bit messages.  Size-independent
-    properties: the three launch forms agree bit for bit on every ciphertext,
+    properties: the plain, batched and rows launch forms agree bit for bit on
+    every ciphertext (16384 ciphertexts exceed the pair form's one pair per CU),
     every one of the 65536 outputs decrypts to the test polynomial at its
     modulus-switched phase (an exact expectation, no tolerance), and a sample
     equals the int128 oracle within the FP64 bound."""
