@@ -61,7 +61,14 @@ FORMS = ("plain", "batched", "rows", "pair")
 # (each poll sleeps ~50 ns and reads the flag, ~0.5-2 s in all; a launch whose
 # workgroups are all resident never comes near this)
 PAIR_SPIN_LIMIT = 1 << 20
+PAIR_RUN_OUT = ("a pair-form blind rotation gave up waiting for its partner workgroup "
+                "(the device did not keep both workgroups of a pair resident); its outputs are "
+                "wrong")
 _PAIR_LAST: dict = {}      # device index -> workspace of the last pair-form call
+# device index -> int32 (1,): OR of the status words of every pair-form call
+# since the last reset (a fill kernel clears each call's own word, so this is
+# what survives later calls)
+_PAIR_STICKY: dict = {}
 
 
 def pair_applicable(degree: int, l: int, batch: int, device=None) -> bool:
@@ -83,6 +90,38 @@ def pair_status(device=None) -> int:
     return 0 if ws is None else int(ws[:4].view(torch.int32).item())
 
 
+def pair_errors(device=None, reset: bool = True) -> int:
+    """OR of the status words of every pair-form call on `device` since the
+    last reset (synchronises the device): bit 0 set means some call's outputs
+    are wrong.  With `reset` the record starts again from zero.  Calls on one
+    stream, or on streams ordered by events, add to it in order; two pair-form
+    calls running at once on unordered streams may race on the word."""
+    dev = torch.cuda.current_device() if device is None else torch.device(device).index
+    st = _PAIR_STICKY.get(dev)
+    if st is None:
+        return 0
+    torch.cuda.synchronize(dev)
+    word = int(st.item())
+    if reset:
+        st.zero_()
+    return word
+
+
+def pair_error_word(device=None):
+    """The device tensor behind pair_errors (None before the first pair-form
+    call on `device`), for callers that copy it to the host asynchronously."""
+    dev = torch.cuda.current_device() if device is None else torch.device(device).index
+    return _PAIR_STICKY.get(dev)
+
+
+def check_pair_errors(device=None) -> None:
+    """Raises TFHEInternalError if a pair-form call on `device` since the last
+    check had a partner wait run out (its outputs are wrong), and starts the
+    record again.  Synchronises with the device."""
+    if pair_errors(device, reset=True) & 1:
+        raise _core.TFHEInternalError(PAIR_RUN_OUT)
+
+
 def _pair_call(ct, bsk, arr, rot, idx, out, batch, log_beta, log_q, l, degree, slots=None,
                slot_count=0, spin_limit=None):
     ws_bytes = int(_core.lib().bslstm_mul_bootstrap_pair_workspace_bytes(batch, l, degree))
@@ -102,7 +141,12 @@ def _pair_call(ct, bsk, arr, rot, idx, out, batch, log_beta, log_q, l, degree, s
             _core.stream_of(ct), ct.data_ptr(), ct.numel(), bsk.data_ptr(), arr, slots.data_ptr(),
             slot_count, idx.data_ptr(), idx.numel(), out.data_ptr(), batch, log_beta, log_q, l,
             degree, ws.data_ptr(), ws_bytes, spin))
-    _PAIR_LAST[ct.device.index] = ws
+    dev = ct.device.index
+    _PAIR_LAST[dev] = ws
+    st = _PAIR_STICKY.get(dev)
+    if st is None:
+        st = _PAIR_STICKY[dev] = torch.zeros(1, dtype=torch.int32, device=ct.device)
+    st.bitwise_or_(ws[:4].view(torch.int32))
 
 
 SPLIT_CHUNK = 512      # ciphertexts per chunk of the split form's step scratch

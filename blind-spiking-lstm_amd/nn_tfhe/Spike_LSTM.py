@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from cuTFHE import torus
-from cuTFHE.multiply import log2_int
+from cuTFHE.multiply import check_pair_errors, log2_int
 from schemas.RLWE_jax import get_delta
 from schemas.polynomial_jax import weights_to_polynomial
 
@@ -248,6 +248,7 @@ class CipherSpikeLSTM:
             Hs.append(H)
             Cs.append(C)
         Hs, Cs = torch.stack(Hs, dim=1), torch.stack(Cs, dim=1)                # (S, T, 2, N)
+        check_pair_errors(A.device)       # wrong states from a run-out pair wait raise here
 
         def pair(t):
             return (t[..., 0, :], t[..., 1, :]) if not single else (t[0, ..., 0, :], t[0, ..., 1, :])

@@ -18,6 +18,7 @@ from __future__ import annotations
 import torch
 
 from cuTFHE import torus
+from cuTFHE.multiply import check_pair_errors
 
 
 class SequenceEngine:
@@ -109,4 +110,7 @@ class SequenceEngine:
         for rank, i in enumerate(order):
             inv[i] = rank
         idx = torch.as_tensor(inv, device=dev)
+        # a pair-form blind rotation whose partner wait ran out left wrong
+        # states: fail here rather than return them (one synchronisation)
+        check_pair_errors(dev)
         return H[:S].index_select(0, idx), C[:S].index_select(0, idx)

@@ -29,7 +29,8 @@ import numpy as np
 import torch
 
 from cuTFHE import torus
-from cuTFHE.multiply import FORMS, log2_int, mul_bootstrap
+from cuTFHE._core import TFHEInternalError
+from cuTFHE.multiply import FORMS, PAIR_RUN_OUT, log2_int, mul_bootstrap, pair_error_word
 from nn_tfhe.Spike_LSTM import CipherSpikeLSTM
 
 
@@ -218,6 +219,8 @@ class EncryptedLSTMServer:
         self.ticks = 0
         self.bootstraps = 0
         self.tick_events: list = []     # (start, end) HIP events of every tick
+        # the pair-form error record as of the last tick, copied without a wait
+        self._pair_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
 
     def submit(self, req: Request):
         self.waiting.append(req)
@@ -304,6 +307,9 @@ class EncryptedLSTMServer:
 
         def settle():
             pending[0].synchronize()
+            if int(self._pair_host[0]) & 1:
+                # a blind rotation of some tick up to this one left wrong states
+                raise TFHEInternalError(PAIR_RUN_OUT)
             end = clock() - t0
             for r in pending[1]:
                 r.done = end
@@ -324,6 +330,9 @@ class EncryptedLSTMServer:
             n_done = len(self.finished)
             stepped = self.tick(now)
             if stepped:
+                word = pair_error_word(self.pool.device)
+                if word is not None:
+                    self._pair_host.copy_(word, non_blocking=True)
                 ev = torch.cuda.Event()
                 ev.record()
                 if pending is not None:

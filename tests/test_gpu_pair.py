@@ -139,15 +139,26 @@ def test_pair_tiny_wait_bound_ends_and_next_call_is_correct(cuda_device, mult):
     idx = dev(O.rotation_indices(case.a, 32, 2048, 3, 1))
     bsk = dev(case.bsk_fft_k)
     lut = dev(case.lut)
+    from cuTFHE import _core
+    mult.pair_errors()                               # start the error record afresh
     mult.mul_bootstrap(lut, bsk, None, idx, 7, 32, 3, form="pair", spin_limit=1)
     torch.cuda.synchronize()
     forced = mult.pair_status()
     print(f"status word after spin_limit=1: {forced:#x}")
     assert forced & ~3 == 0
     r_pair = host(mult.mul_bootstrap(lut, bsk, None, idx, 7, 32, 3, form="pair"))
-    assert mult.pair_status() & 1 == 0
+    last = mult.pair_status()
+    assert last & 1 == 0
     r_rows = host(mult.mul_bootstrap(lut, bsk, None, idx, 7, 32, 3, form="rows"))
     np.testing.assert_array_equal(r_pair, r_rows)
+    # the record keeps the forced call's word after the clean call cleared its own
+    assert mult.pair_errors(reset=False) == forced | last
+    if forced & 1:
+        with pytest.raises(_core.TFHEInternalError, match="partner"):
+            mult.check_pair_errors()
+    else:
+        mult.check_pair_errors()
+    assert mult.pair_errors() == 0                   # the check started the record again
 
 
 def test_pair_rejects_outside_its_range(cuda_device, mult):
