@@ -290,7 +290,7 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
         // ---- gadget product with the Galois key of this step; group h
         // transforms the level pairs h, h + RG, ..., each pair's digits read
         // straight from bit fields of the class and its carry word
-        // (digit_from_carries)
+        // (digit_level / digit_at_level)
         const cplx *key = gkey + (size_t)st * l * 2 * M;
         // digits of level pair `pair`, their transforms, and the products with the
         // key rows of those levels added into A (in level order)
@@ -299,11 +299,23 @@ __global__ __launch_bounds__(N / 8 * RG, (RG == 1 && Wave<N>::OK) ? 2 : 1) void 
             const int lev = 2 * pair;
             double dg[2][8];
 #pragma unroll
-            for (int i = 0; i < 8; i++)
+            for (int c = 0; c < 2; c++) {
+                // one uniform branch per level, none per coefficient
+                const DigitLevel dl = digit_level(lev + c, l, lb, lq);
+                if (dl.kind == 1) {
 #pragma unroll
-                for (int c = 0; c < 2; c++)
-                    dg[c][i] = lev + c < l ? (double)digit_from_carries((uint64_t)r[i], cv[i], lev + c, lb, lq)
-                                           : 0.0;
+                    for (int i = 0; i < 8; i++)
+                        dg[c][i] = (double)digit_at_level<1>(dl, (uint64_t)r[i], cv[i]);
+                } else if (dl.kind == 2) {
+#pragma unroll
+                    for (int i = 0; i < 8; i++)
+                        dg[c][i] = (double)digit_at_level<2>(dl, (uint64_t)r[i], cv[i]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; i++)
+                        dg[c][i] = 0.0;
+                }
+            }
 #pragma unroll
             for (int c = 0; c < 2; c++)
 #pragma unroll

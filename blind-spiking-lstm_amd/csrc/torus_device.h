@@ -75,7 +75,7 @@ __device__ __forceinline__ int64_t residual_at(int64_t r, int S, int lq)
 //   bit p = u_p & (u_(p+1) | p > ctz(u)),
 // and bit lq-1 is the sign, which the formula gives through the sign-extended
 // bit lq except at lq = 64 (u = 2^63): or-ed in.  One word per coefficient
-// per level loop; each digit is then two fields and two carry bits.
+// per level loop; each digit is then a field and two carry bits.
 __device__ __forceinline__ uint64_t carries_of(uint64_t u, int lq)
 {
     const uint64_t low = u & (0 - u);              // lowest set bit (0 for u = 0)
@@ -83,16 +83,45 @@ __device__ __forceinline__ uint64_t carries_of(uint64_t u, int lq)
     return (u & ((u >> 1) | above)) | (u & (1ULL << (lq - 1)));
 }
 
-__device__ __forceinline__ int digit_from_carries(uint64_t u, uint64_t cv, int j, int lb, int lq)
+// Digit j for a run of coefficients: the level's shifts and masks are the
+// same for every coefficient (wave-uniform), so they are worked out once
+// (digit_level) and each digit (digit_at_level) is two 64-bit shifts and four
+// 32-bit operations (v_bfi, v_bfe_i32, and, add), with no branch.  With
+// w = S - S1, uu = u >> S1 (the field in bits [0, w)) and cc = cv >> (S1 - 1)
+// (up(S1) in bit 0, up(S) in bit w), inserting bits [0, w) of uu into cc and
+// reading bits [0, w] as a signed number gives field - up(S) * 2^w; bit 0 of
+// cc adds up(S1).  At S1 = 0, up(S1) = 0 and cc = cv << 1.  Needs w <= 30
+// (log2 of the base).
+struct DigitLevel {
+    int kind;           // 0: every digit is 0 (j >= l or S <= 0); 1: S1 > 0; 2: S1 = 0
+    int s1, w;
+    uint32_t fmask;     // bits [0, w)
+};
+
+__device__ __forceinline__ DigitLevel digit_level(int j, int l, int lb, int lq)
 {
     const int S = lq - j * lb;
-    if (S <= 0)
-        return 0;
+    if (j >= l || S <= 0)
+        return {0, 0, 0, 0u};
     const int S1 = S > lb ? S - lb : 0;
-    const int field = (int)((u >> S1) & ((1ULL << (S - S1)) - 1));
-    const int lo = S1 > 0 ? (int)((cv >> (S1 - 1)) & 1) : 0;
-    const int hi = (int)((cv >> (S - 1)) & 1);
-    return field + lo - (hi << (S - S1));
+    const int w = S - S1;
+    return {S1 > 0 ? 1 : 2, S1, w, (1u << w) - 1u};
+}
+
+template <int KIND>
+__device__ __forceinline__ int digit_at_level(const DigitLevel &d, uint64_t u, uint64_t cv)
+{
+    static_assert(KIND == 1 || KIND == 2, "kind 0 digits are 0");
+    uint32_t uu, cc;
+    if constexpr (KIND == 1) {
+        uu = (uint32_t)(u >> d.s1);
+        cc = (uint32_t)(cv >> (d.s1 - 1));
+    } else {
+        uu = (uint32_t)u;
+        cc = (uint32_t)cv << 1;
+    }
+    const uint32_t x = (uu & d.fmask) | (cc & ~d.fmask);
+    return __builtin_amdgcn_sbfe((int)x, 0, d.w + 1) + (int)(cc & 1u);
 }
 
 // Torus class of a rounded FP64 transform output (jax_inversefourier's

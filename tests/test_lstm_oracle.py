@@ -88,28 +88,35 @@ def _carries(x, log_q):
 
 
 def _digit_from_bits(x, j, log_beta, log_q):
-    """The closed form csrc/torus_device.h's digit_from_carries computes for
-    level j of the float-rule recursion: with S = log_q - j*log_beta and S1 =
-    max(S - log_beta, 0), d_j = bits [S1, S) of x + up(S1) - up(S) *
-    2^(S - S1), the carries read from the carry word (bit S-1 is up(S))."""
+    """csrc/torus_device.h's digit_level / digit_at_level for level j of the
+    float-rule recursion, on 32-bit words as the device has them: with S =
+    log_q - j*log_beta, S1 = max(S - log_beta, 0) and w = S - S1, uu = x >> S1
+    holds the field in bits [0, w) and cc = cv >> (S1 - 1) (cv << 1 at S1 = 0)
+    holds up(S1) in bit 0 and up(S) in bit w; the field inserted into cc and
+    read as a signed (w+1)-bit number is field - up(S) * 2^w, and bit 0 of cc
+    adds up(S1) (bit p of the carry word cv is up(p + 1))."""
     u = x.astype(np.uint64)
     cv = _carries(x, log_q)
     S = log_q - j * log_beta
     if S <= 0:
         return np.zeros(u.shape, np.int64)
     S1 = max(S - log_beta, 0)
-    field = ((u >> np.uint64(S1)) & np.uint64((1 << (S - S1)) - 1)).astype(np.int64)
-    lo = ((cv >> np.uint64(S1 - 1)) & np.uint64(1)).astype(np.int64) if S1 > 0 else 0
-    hi = ((cv >> np.uint64(S - 1)) & np.uint64(1)).astype(np.int64)
-    return field + lo - (hi << (S - S1))
+    w = S - S1
+    m32 = np.uint64(0xFFFFFFFF)
+    uu = (u >> np.uint64(S1)) & m32
+    cc = ((cv >> np.uint64(S1 - 1)) if S1 > 0 else (cv << np.uint64(1))) & m32
+    fm = np.uint64((1 << w) - 1)
+    ins = ((uu & fm) | (cc & ~fm)) & np.uint64((1 << (w + 1)) - 1)     # bits [0, w]
+    signed = ins.astype(np.int64) - ((ins >> np.uint64(w)).astype(np.int64) << (w + 1))
+    return signed + (cc & np.uint64(1)).astype(np.int64)
 
 
 @pytest.mark.parametrize("log_q,log_beta,l", [(64, 2, 20), (64, 13, 2), (64, 8, 8), (32, 7, 3),
                                               (48, 4, 12), (20, 3, 7), (24, 5, 6), (64, 2, 32),
                                               (32, 4, 8), (16, 1, 16), (12, 5, 4)])
 def test_digits_from_bit_fields_equal_recursion(log_q, log_beta, l):
-    """The trace-packing and key-switch kernels read each digit of the float
-    rule straight from bit fields of the centred class (no recursion): equal
+    """The trace-packing kernel reads each digit of the float rule straight
+    from bit fields of the centred class and its carry word (no recursion): equal
     to the oracle's digits at every level, with round-half ties planted at
     every level and the all-zero / sign-boundary classes included.  A carry
     that ignored the tie bit, the sticky lower bits, or the sign at the top
