@@ -236,7 +236,7 @@ def run_form_sweep():
     inputs (the time does not depend on their values); 3-launch minimum."""
     import torch
     sys.path.insert(0, PKG)
-    from cuTFHE.multiply import mul_bootstrap, pair_applicable
+    from cuTFHE.multiply import check_pair_errors, mul_bootstrap, pair_applicable
     dev = torch.device("cuda:0")
     N = 2048
     out = {}
@@ -264,6 +264,7 @@ def run_form_sweep():
                     ts.append(a.elapsed_time(b))
                 ms[form] = round(min(ts), 3)
             rows[str(B)] = ms
+        check_pair_errors()
         out[name] = rows
         del bsk
     return {"unit": "ms per launch", "by_batch": out}
@@ -386,6 +387,7 @@ def run_lstm(args):
     import parameters_crypto as PC
     import setup_crypto as SC
     from nn_tfhe.Spike_LSTM import CipherSpikeLSTM
+    from cuTFHE.multiply import check_pair_errors
     from nn_tfhe.engine import SequenceEngine
     from schemas.RLWE_jax import encrypt
 
@@ -432,6 +434,7 @@ def run_lstm(args):
             H, C, lut = lstm.step_pipelined(H, C, x[t + 2], lut)
             evs[t][1].record(stream)
         torch.cuda.synchronize()
+        check_pair_errors()        # a run-out pair wait would have left wrong states
         ms = np.array([a.elapsed_time(e) for a, e in evs])
         key = "latency_batch1" if S == 1 else f"throughput_batch{S}"
         out[key] = {"sequences": S, "steps": steps, "p50_ms": round(float(np.percentile(ms, 50)), 3),
