@@ -734,6 +734,34 @@ def cpu_baseline_child(sample: int, seed: int = 1):
     print(json.dumps({"value": sample / dt, "seconds": dt}))
 
 
+def cpu_config1_child(sample: int, seed: int = 1):
+    """One worker of the config-1 all-cores figure: the oracle's NumPy
+    restatement of the PBS, extraction and key switch of `sample`
+    ciphertexts at n=630, N=1024, single-threaded, on random keys and inputs
+    of the same shapes (the time does not depend on their values; the
+    one-core run checks the decryptions)."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import tfhe_oracle as O
+    c = PBS_KS_CFG
+    n, N, L, lb, cl, lq = c["n"], c["N"], c["L"], c["log_beta"], c["collapse"], c["log_q"]
+    rng = np.random.default_rng(seed)
+    fft_k = rng.standard_normal((n // cl, 2, L, 2, 2 ** cl, N // 2, 2)).view(np.complex128)[..., 0]
+    fft_k *= 2.0 ** 30
+    ka = rng.integers(-2 ** 31, 2 ** 31, size=(c["ks_l"], N, n))
+    kb = rng.integers(-2 ** 31, 2 ** 31, size=(c["ks_l"], N))
+    a = rng.integers(-2 ** 31, 2 ** 31, size=(sample, n))
+    b = rng.integers(-2 ** 31, 2 ** 31, size=sample)
+    lut = rng.integers(-2 ** 31, 2 ** 31, size=(2, N))
+    t0 = time.perf_counter()
+    idx = O.rotation_indices(a, lq, N, cl, 1)
+    br = O.blind_rotate_f64(np.broadcast_to(lut, (sample, 2, N)), fft_k, idx, lb, lq, L)
+    m_, b_ = O.cuboot_merge_tail(br, b, lq, 1)
+    O.key_switch_lwe(ka, kb, m_[:, 0], b_[:, 0], lq, c["ks_log_beta"], c["ks_l"])
+    dt = time.perf_counter() - t0
+    print(json.dumps({"value": sample / dt, "seconds": dt}))
+
+
 # the reference's CPU-runnable case: 256 LWE ciphertexts, n=630, N=1024, l=3,
 # log2 Bg=7, collapse 3, identity LUT on 3-bit messages with a padding bit,
 # then sample extract and key switch back to n (base 2^4, l=5)
@@ -824,7 +852,19 @@ def run_pbs_keyswitch(cpu: bool):
         out["cpu_restatement"] = {"sample": Bc, "seconds": round(dt, 2),
                                   "bootstraps_per_s": round(Bc / dt, 2), "cores": 1,
                                   "messages_correct": float((dec_c == msg[:Bc]).mean()),
-                                  "gpu_equals_restatement": bool(np.array_equal(dec[:Bc], dec_c))}
+                                  "gpu_equals_restatement": bool(np.array_equal(dec[:Bc], dec_c)),
+                                  "cpu_model": cpu_model()}
+        # all cores: one single-threaded worker per host core at once
+        P, per = host_cores(), 32
+        try:
+            many = run_cpu_workers("--cpu-config1-child", P, per)
+            wall = max(d["seconds"] for d in many)
+            out["cpu_restatement"]["all_cores"] = {
+                "bootstraps_per_s": round(P * per / wall, 2), "cores": P,
+                "sample": f"{per} ciphertexts per process, {P} processes at once, slowest "
+                          f"{wall:.1f} s (random keys and inputs of the same shapes)"}
+        except (RuntimeError, subprocess.TimeoutExpired) as e:
+            out["cpu_restatement"]["all_cores"] = {"error": str(e)}
     return out
 
 
@@ -981,28 +1021,32 @@ def single_thread_env() -> dict:
     return env
 
 
+def run_cpu_workers(flag: str, n_procs: int, per: int) -> list:
+    """n_procs single-threaded worker processes of bench.py (`flag` = the
+    worker's switch), started together, `per` ciphertexts each; returns
+    their JSON lines."""
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), flag, str(per),
+                               "--cpu-baseline-seed", str(1 + i)],
+                              env=single_thread_env(), stdout=subprocess.PIPE, text=True)
+             for i in range(n_procs)]
+    outs = []
+    for p in procs:
+        o, _ = p.communicate(timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError("CPU baseline worker failed")
+        outs.append(json.loads(o.strip().splitlines()[-1]))
+    return outs
+
+
 def cpu_baseline(sample: int):
     """The oracle's NumPy FP64 restatement of the reference algorithm on a
     bounded sample of the headline workload: one single-threaded process
     (1 core), then one process per host core at once (all cores)."""
-    def run(n_procs, per):
-        procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
-                                   str(per), "--cpu-baseline-seed", str(1 + i)],
-                                  env=single_thread_env(), stdout=subprocess.PIPE, text=True)
-                 for i in range(n_procs)]
-        outs = []
-        for p in procs:
-            o, _ = p.communicate(timeout=900)
-            if p.returncode != 0:
-                raise RuntimeError("CPU baseline worker failed")
-            outs.append(json.loads(o.strip().splitlines()[-1]))
-        return outs
-
     try:
-        one = run(1, sample)[0]
+        one = run_cpu_workers("--cpu-baseline-child", 1, sample)[0]
         P = host_cores()
         per = max(8, sample // 4)
-        many = run(P, per)
+        many = run_cpu_workers("--cpu-baseline-child", P, per)
         wall = max(d["seconds"] for d in many)
         all_rate = P * per / wall
     except (RuntimeError, subprocess.TimeoutExpired) as e:
@@ -1054,11 +1098,15 @@ def main():
     ap.add_argument("--split", type=int, default=1,
                     help="also time the split form (exact int8 MFMA) at the headline workload")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
+    ap.add_argument("--cpu-config1-child", type=int, default=0)
     ap.add_argument("--cpu-baseline-seed", type=int, default=1)
     ap.add_argument("--restate-child", type=str, default="")
     args = ap.parse_args()
     if args.cpu_baseline_child:
         cpu_baseline_child(args.cpu_baseline_child, args.cpu_baseline_seed)
+        return
+    if args.cpu_config1_child:
+        cpu_config1_child(args.cpu_config1_child, args.cpu_baseline_seed)
         return
     if args.restate_child:
         p, lo, hi, lb, lq, L = args.restate_child.split(",")
