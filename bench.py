@@ -739,7 +739,8 @@ def cpu_config1_child(sample: int, seed: int = 1):
     restatement of the PBS, extraction and key switch of `sample`
     ciphertexts at n=630, N=1024, single-threaded, on random keys and inputs
     of the same shapes (the time does not depend on their values; the
-    one-core run checks the decryptions)."""
+    one-core run checks the decryptions); two warm-ups, then the median of
+    five repetitions."""
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import tfhe_oracle as O
@@ -753,12 +754,21 @@ def cpu_config1_child(sample: int, seed: int = 1):
     a = rng.integers(-2 ** 31, 2 ** 31, size=(sample, n))
     b = rng.integers(-2 ** 31, 2 ** 31, size=sample)
     lut = rng.integers(-2 ** 31, 2 ** 31, size=(2, N))
-    t0 = time.perf_counter()
-    idx = O.rotation_indices(a, lq, N, cl, 1)
-    br = O.blind_rotate_f64(np.broadcast_to(lut, (sample, 2, N)), fft_k, idx, lb, lq, L)
-    m_, b_ = O.cuboot_merge_tail(br, b, lq, 1)
-    O.key_switch_lwe(ka, kb, m_[:, 0], b_[:, 0], lq, c["ks_log_beta"], c["ks_l"])
-    dt = time.perf_counter() - t0
+
+    def pbs(lo, hi):
+        idx = O.rotation_indices(a[lo:hi], lq, N, cl, 1)
+        br = O.blind_rotate_f64(np.broadcast_to(lut, (hi - lo, 2, N)), fft_k, idx, lb, lq, L)
+        m_, b_ = O.cuboot_merge_tail(br, b[lo:hi], lq, 1)
+        O.key_switch_lwe(ka, kb, m_[:, 0], b_[:, 0], lq, c["ks_log_beta"], c["ks_l"])
+
+    for _ in range(2):                          # warm-ups
+        pbs(0, min(4, sample))
+    reps = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        pbs(0, sample)
+        reps.append(time.perf_counter() - t0)
+    dt = float(np.median(reps))
     print(json.dumps({"value": sample / dt, "seconds": dt}))
 
 
@@ -840,29 +850,47 @@ def run_pbs_keyswitch(cpu: bool):
     if cpu:
         from threadpoolctl import threadpool_limits
         Bc = c["cpu_sample"]                    # a bounded sample: the first Bc ciphertexts
+        rep = Bc // 4                           # one repetition: a quarter of the sample
+
+        def restate(lo, hi):
+            idx = O.rotation_indices(a[lo:hi], lq, N, cl, 1)
+            br = O.blind_rotate_f64(np.broadcast_to(lut, (hi - lo, 2, N)), fft_k, idx, lb, lq, L)
+            m_, b_ = O.cuboot_merge_tail(br, b[lo:hi], lq, 1)
+            return O.key_switch_lwe(ka, kb, m_[:, 0], b_[:, 0], lq, c["ks_log_beta"], c["ks_l"])
+
         with threadpool_limits(limits=1):
-            t0 = time.perf_counter()
-            idx = O.rotation_indices(a[:Bc], lq, N, cl, 1)
-            br = O.blind_rotate_f64(np.broadcast_to(lut, (Bc, 2, N)), fft_k, idx, lb, lq, L)
-            m_, b_ = O.cuboot_merge_tail(br, b[:Bc], lq, 1)
-            ra, rb = O.key_switch_lwe(ka, kb, m_[:, 0], b_[:, 0], lq, c["ks_log_beta"], c["ks_l"])
-            dt = time.perf_counter() - t0
-        ph = O.centered_mod(np.asarray(rb, np.int64).reshape(-1) - np.asarray(ra, np.int64) @ s, lq)
+            for _ in range(2):                  # warm-ups
+                restate(0, 4)
+            reps, parts = [], []
+            for k in range(5):                  # the four quarters, then the first again
+                lo = (k % 4) * rep
+                t0 = time.perf_counter()
+                r_ = restate(lo, lo + rep)
+                reps.append(time.perf_counter() - t0)
+                if k < 4:
+                    parts.append(r_)
+        dt = float(np.median(reps))
+        ra = np.concatenate([np.asarray(p_[0], np.int64) for p_ in parts])
+        rb = np.concatenate([np.asarray(p_[1], np.int64).reshape(-1) for p_ in parts])
+        ph = O.centered_mod(rb - ra @ s, lq)
         dec_c = np.mod(np.rint(ph / delta).astype(np.int64), 1 << (c["msg_bits"] + 1))
-        out["cpu_restatement"] = {"sample": Bc, "seconds": round(dt, 2),
-                                  "bootstraps_per_s": round(Bc / dt, 2), "cores": 1,
+        out["cpu_restatement"] = {"sample": Bc, "timing": f"median of 5 repetitions of {rep} "
+                                  f"ciphertexts after 2 warm-ups",
+                                  "seconds_per_repetition": round(dt, 3),
+                                  "bootstraps_per_s": round(rep / dt, 2), "cores": 1,
                                   "messages_correct": float((dec_c == msg[:Bc]).mean()),
                                   "gpu_equals_restatement": bool(np.array_equal(dec[:Bc], dec_c)),
                                   "cpu_model": cpu_model()}
         # all cores: one single-threaded worker per host core at once
-        P, per = host_cores(), 32
+        P, per = host_cores(), 16
         try:
             many = run_cpu_workers("--cpu-config1-child", P, per)
             wall = max(d["seconds"] for d in many)
             out["cpu_restatement"]["all_cores"] = {
                 "bootstraps_per_s": round(P * per / wall, 2), "cores": P,
-                "sample": f"{per} ciphertexts per process, {P} processes at once, slowest "
-                          f"{wall:.1f} s (random keys and inputs of the same shapes)"}
+                "sample": f"{per} ciphertexts per process, {P} processes at once, each the "
+                          f"median of 5 repetitions after 2 warm-ups; slowest {wall:.2f} s "
+                          f"(random keys and inputs of the same shapes)"}
         except (RuntimeError, subprocess.TimeoutExpired) as e:
             out["cpu_restatement"]["all_cores"] = {"error": str(e)}
     return out
