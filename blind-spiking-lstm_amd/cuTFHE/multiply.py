@@ -67,7 +67,9 @@ PAIR_RUN_OUT = ("a pair-form blind rotation gave up waiting for its partner work
 _PAIR_LAST: dict = {}      # device index -> workspace of the last pair-form call
 # device index -> int32 (1,): OR of the status words of every pair-form call
 # since the last reset (a fill kernel clears each call's own word, so this is
-# what survives later calls)
+# what survives later calls).  Made by the first pair-form call on the device;
+# if that call is inside a graph capture, the graph also holds its zero fill
+# and each replay starts the record afresh (an eager call first avoids that).
 _PAIR_STICKY: dict = {}
 
 
